@@ -1,0 +1,192 @@
+/*
+ * vmc_afqmc.h -- C ABI of the MI355X phaseless-AFQMC walker step.
+ *
+ * One context (afq_ctx) holds one rank's whole walker population, the
+ * Hamiltonian tensors and the trial on one GPU.  Every call is stream-ordered
+ * on the context's HIP stream; a context is not re-entrant (one host thread at
+ * a time).  All calls return AFQ_OK (0) or a nonzero status; the message of the
+ * last failure is available from afq_last_error().  The library never calls
+ * exit(); numeric failures of single walkers are reported through weights and
+ * flags exactly as the reference does (weight factor 0).
+ *
+ * Data conventions (host buffers unless a name ends in _dev):
+ *   - complex numbers are interleaved (re, im) doubles, i.e. the memory layout
+ *     of std::complex<double> / double _Complex;
+ *   - matrices are row-major unless stated otherwise;
+ *   - walker determinants use the reference's serial layout
+ *     (DQMC/DQMCWalker.cpp:251-290): for each walker, psi_up row-major
+ *     norb x nalpha, then psi_dn row-major norb x nbeta (mat_size =
+ *     norb*(nalpha+nbeta) complex numbers per walker).
+ *
+ * Reference interfaces replaced (paths relative to the sanshar/VMC tree):
+ *   afq_create          Hamiltonian ctor + floattenCholesky + RHF/UHF left-trial
+ *                       ctor with rotateCholesky  (DQMC/Hamiltonian.cpp:9-42,
+ *                       52-133,179-220; DQMC/RHF.cpp:9-17; DQMC/UHF.cpp:9-21)
+ *   afq_set_nchol_ene   Hamiltonian::setNcholEne (DQMC/Hamiltonian.cpp:45-48)
+ *   afq_prep_prop       DQMCWalker::prepPropR (DQMC/DQMCWalker.cpp:32-103)
+ *   afq_set_walkers     DQMCWalker::setDet(serial, overlap) (DQMC/DQMCWalker.cpp:251-269)
+ *   afq_get_walkers     DQMCWalker::getDet (DQMC/DQMCWalker.cpp:272-290)
+ *   afq_propagate       walker loop of calcMixedEstimatorLongProp with
+ *                       DQMCWalker::propagatePhaselessRG
+ *                       (DQMC/MixedEstimator.cpp:461-468; DQMC/DQMCWalker.cpp:441-503)
+ *   afq_orthogonalize   DQMCWalker::orthogonalize loop (DQMC/MixedEstimator.cpp:478-480;
+ *                       DQMC/DQMCWalker.cpp:295-326)
+ *   afq_measure         DQMCWalker::hamAndOverlap loop with the E_L clip
+ *                       (DQMC/MixedEstimator.cpp:489-533; DQMC/RHF.cpp:104-131;
+ *                       DQMC/UHF.cpp:75-106)
+ *   afq_force_bias      DQMCWalker::forceBias (DQMC/DQMCWalker.cpp:569-576;
+ *                       DQMC/RHF.cpp:42-61; DQMC/UHF.cpp:45-65)
+ *   afq_overlap         DQMCWalker::overlap (DQMC/DQMCWalker.cpp:559-566)
+ *   afq_comb            reconfiguration (DQMC/MixedEstimator.cpp:559-635)
+ *   afq_run_long_prop   calcMixedEstimatorLongProp (DQMC/MixedEstimator.cpp:309-699)
+ */
+#ifndef VMC_AFQMC_H
+#define VMC_AFQMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFQ_ABI_VERSION 1
+
+/* status codes */
+#define AFQ_OK 0
+#define AFQ_ERR_ARG 1          /* bad argument / shape */
+#define AFQ_ERR_DEVICE 2       /* HIP runtime failure */
+#define AFQ_ERR_UNSUPPORTED 3  /* shape or mode this build does not support */
+#define AFQ_ERR_STATE 4        /* call out of order (e.g. propagate before prep_prop) */
+#define AFQ_ERR_NUMERIC 5      /* NaN/Inf local energy (reference exits, we report) */
+
+/* arithmetic used by a stage (afq_params.prec_*) */
+#define AFQ_PREC_FP64 0    /* plain double precision                          */
+#define AFQ_PREC_FP32 1    /* float operands, float accumulation              */
+#define AFQ_PREC_BF16X3 2  /* bf16 hi/lo split, 3 MFMA passes, fp32 accumulate */
+#define AFQ_PREC_BF16 3    /* one bf16 pass, fp32 accumulate                  */
+#define AFQ_PREC_FP8 4     /* e4m3 with per-Cholesky-vector scale + fp32 residual */
+
+/* local-energy evaluation (afq_measure mode) */
+#define AFQ_ENERGY_EXACT 0      /* exact E_L on every live walker                   */
+#define AFQ_ENERGY_SURROGATE 1  /* surrogate exchange on all + exact on a subset, */
+                                /* two-level control variate (needs a surrogate)    */
+
+/* Hamiltonian in the fields of the reference's FCIDUMP_chol
+ * (utils/integral.cpp:589-661, scripts/QMCUtils.py:1067-1075). */
+typedef struct afq_ham {
+  int32_t norb;        /* N                                  */
+  int32_t nalpha;      /* electrons of spin up               */
+  int32_t nbeta;       /* electrons of spin down             */
+  int32_t nchol;       /* G, number of Cholesky vectors       */
+  double ecore;        /* energy_core                        */
+  const double* h1;     /* hcore, N x N                      */
+  const double* h1_mod; /* hcore_mod = hcore - 1/2 sum_g L_g L_g, N x N */
+  const double* chol;   /* G x N x N, row g = vec(L_g), each L_g symmetric */
+} afq_ham;
+
+/* Left trial (RHF or UHF, DQMC/RHF.h, DQMC/UHF.h).  Orbitals are real. */
+typedef struct afq_trial {
+  int32_t rhf;          /* 1: RHF (phi_dn ignored, nalpha == nbeta required) */
+  const double* phi_up; /* N x nalpha, row-major                            */
+  const double* phi_dn; /* N x nbeta, row-major                             */
+} afq_trial;
+
+typedef struct afq_params {
+  int32_t nwalk;       /* walkers in this context                                  */
+  int32_t walker_rhf;  /* DQMCWalker::rhfQ: one determinant shared by both spins   */
+                       /* (requires nalpha == nbeta; executables/DQMC.cpp:74-82)    */
+  int32_t ref_compat;  /* 1: round L and fields to float in the VHS and mean-field */
+                       /* shifts like the reference (DQMC/DQMCWalker.cpp:73-84,454) */
+  int32_t prec_fb;     /* force bias arithmetic  (AFQ_PREC_*)                     */
+  int32_t prec_vhs;    /* VHS arithmetic                                           */
+  int32_t prec_exchange; /* exchange contraction arithmetic                      */
+  int32_t device;      /* HIP device ordinal                                       */
+  double weight_cap;   /* <= 0: max(100, nwalk/10) (DQMC/MixedEstimator.cpp:452-454) */
+} afq_params;
+
+typedef struct afq_ctx afq_ctx;
+
+/* Message of the last failure on ctx (or of the last failed afq_create when
+ * ctx is NULL).  Valid until the next call on the same context. */
+const char* afq_last_error(const afq_ctx* ctx);
+
+/* Library / ABI version; also forces the shared object to load. */
+int afq_version(void);
+
+/* Create a context: uploads h1, h1_mod, L_g (FP64), the packed lower-triangle
+ * L (floattenCholesky layout) and the half-rotated Cholesky R_{s,g} = Phi_s^T L_g
+ * of the left trial.  Allocates the walker population (all weights 0 until
+ * afq_set_walkers). */
+int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* params,
+               afq_ctx** out);
+int afq_destroy(afq_ctx* ctx);
+
+/* Number of Cholesky vectors used by the local energy (default nchol). */
+int afq_set_nchol_ene(afq_ctx* ctx, int32_t nchol_ene);
+
+/* Mean-field background subtraction and one-body propagator.
+ * ref_up/ref_dn: reference determinant (N x nalpha / N x nbeta, row-major)
+ * used for Gref = ref (ref^T ref)^-1 ref^T unless rdm_up/rdm_dn (N x N) are
+ * given (the spinRDM.0.0.txt path).  ene0 is accepted for interface parity;
+ * the phaseless propagator constant does not depend on it. */
+int afq_prep_prop(afq_ctx* ctx, const double* ref_up, const double* ref_dn,
+                  const double* rdm_up, const double* rdm_dn, double dt, double ene0);
+
+/* Walker population I/O in the serial layout.  overlaps (complex, nwalk) may be
+ * NULL on set: the trial overlap is then computed (DQMCWalker::overlap).
+ * weights may be NULL on set (all 1).  Any output pointer may be NULL on get. */
+int afq_set_walkers(afq_ctx* ctx, const double* dets, const double* overlaps,
+                    const double* weights);
+int afq_get_walkers(afq_ctx* ctx, double* dets, double* overlaps, double* weights);
+int afq_get_weights(afq_ctx* ctx, double* weights);
+int afq_set_weights(afq_ctx* ctx, const double* weights);
+
+/* One phaseless step of the whole population.  draws: nwalk x nchol standard
+ * normals, row w used only if walker w is alive (weight > 1e-8), or NULL to draw
+ * on the device from (seed, walker slot, step) with a counter-based generator.
+ * Applies weight *= factor for live walkers and the weight cap; returns the
+ * per-walker factors in factors (nullable) and the number of walkers zeroed by
+ * the cap in n_capped (nullable).  eshift as in DQMC/MixedEstimator.cpp:475. */
+int afq_propagate(afq_ctx* ctx, const double* draws, uint64_t seed, uint64_t step,
+                  double eshift, double* factors, int64_t* n_capped);
+
+/* Householder re-orthonormalisation of every walker (Eigen's sign convention,
+ * R_kk = -sign(Re x_0) |x|), trialOverlap /= prod diag R. */
+int afq_orthogonalize(afq_ctx* ctx);
+
+/* Local energies of every walker with weight != 0 (others 0).
+ * mode AFQ_ENERGY_EXACT: eloc[w] = Re(E*O/O) + delta.  ham_ovlp (nullable):
+ * per walker {E*O, O} as two complex numbers (hamAndOverlap).  No clipping here;
+ * see afq_clip_energies. */
+int afq_measure(afq_ctx* ctx, int32_t mode, double delta, double* eloc, double* ham_ovlp);
+
+/* E_L of arbitrary determinants against the left trial (init-time helper used
+ * for the reference energy and the Cholesky truncation search,
+ * DQMC/MixedEstimator.cpp:352-384).  dets: ndet determinants in serial layout;
+ * out: ndet x {E*O, O} complex. */
+int afq_ham_and_overlap(afq_ctx* ctx, const double* dets, int32_t ndet,
+                        int32_t nchol_ene, double* out);
+
+/* Parity hooks: force bias (nwalk x nchol complex) and trial overlaps of the
+ * current population, computed from the current determinants. */
+int afq_force_bias(afq_ctx* ctx, double* fb);
+int afq_overlap(afq_ctx* ctx, double* overlaps);
+
+/* Comb reconfiguration with the uniform zeta (DQMC/MixedEstimator.cpp:595-622):
+ * new walker w is old walker lower_bound(cum, (w+zeta)/W * sum). Sets every
+ * weight to sum/W.  total_weight (nullable) receives sum. */
+int afq_comb(afq_ctx* ctx, double zeta, double* total_weight);
+
+/* Per-stage device time (ms) accumulated since the last reset, in the order
+ * force_bias, fields, vhs, propagate, green, weight, ortho, energy, comb. */
+int afq_stage_times(afq_ctx* ctx, double* ms9, int32_t reset);
+
+/* Synchronise the context's stream. */
+int afq_sync(afq_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VMC_AFQMC_H */

@@ -1,0 +1,219 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_afqmc.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg load this, always as the checker or the timed CPU baseline.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from vmc_amd.abi import AfqHam, AfqTrial, DP, dptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle_afqmc.so")
+
+
+class RunParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_int64), ("dt", C.c_double), ("nsteps", C.c_int32),
+        ("nsweeps", C.c_int32), ("ortho_steps", C.c_int32), ("burn_iter", C.c_int32),
+        ("nwalk", C.c_int32), ("nranks", C.c_int32), ("walker_rhf", C.c_int32),
+        ("ref_compat", C.c_int32), ("cholesky_threshold", C.c_double),
+        ("ene0_guess", C.c_double), ("weight_cap", C.c_double), ("max_steps", C.c_int32),
+    ]
+
+
+class RunResult(C.Structure):
+    _fields_ = [("ref_energy", C.c_double), ("nchol_ene", C.c_int32), ("delta", C.c_double),
+                ("n_large_deviations", C.c_int64), ("prop_time", C.c_double),
+                ("ene_time", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        L = C.CDLL(ORACLE_SO)
+        vp = C.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [C.POINTER(AfqHam), C.POINTER(AfqTrial), C.c_int, C.c_int, C.c_int]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_nchol_ene.argtypes = [vp, C.c_int]
+        L.orc_prep_prop.argtypes = [vp, DP, DP, DP, DP, C.c_double, C.c_double]
+        L.orc_get_prop_data.argtypes = [vp, DP, DP, DP]
+        L.orc_set_walkers.argtypes = [vp, DP, DP, DP]
+        L.orc_get_walkers.argtypes = [vp, DP, DP, DP]
+        L.orc_propagate_walker.restype = C.c_double
+        L.orc_propagate_walker.argtypes = [vp, C.c_int, DP, C.c_double, DP, DP]
+        L.orc_propagate.argtypes = [vp, DP, C.c_double, C.c_double, DP, C.POINTER(C.c_int64)]
+        L.orc_orthogonalize.argtypes = [vp]
+        L.orc_orthogonalize_walker.argtypes = [vp, C.c_int]
+        L.orc_ham_and_overlap_walker.argtypes = [vp, C.c_int, DP]
+        L.orc_force_bias_walker.argtypes = [vp, C.c_int, DP]
+        L.orc_overlap_walker.argtypes = [vp, C.c_int, DP]
+        L.orc_green_walker.argtypes = [vp, C.c_int, C.c_int, DP, DP]
+        L.orc_energy_parts_walker.argtypes = [vp, C.c_int, DP, DP, DP]
+        L.orc_ham_and_overlap_dets.argtypes = [vp, DP, C.c_int, C.c_int, DP]
+        L.orc_comb.argtypes = [vp, C.c_double, DP]
+        L.orc_run_long_prop.restype = C.c_int
+        L.orc_run_long_prop.argtypes = [C.POINTER(AfqHam), C.POINTER(AfqTrial), DP, DP, DP, DP,
+                                        C.POINTER(RunParams), DP, DP, C.POINTER(RunResult)]
+        L.orc_blocking.argtypes = [DP, DP, C.c_int, DP]
+        L.orc_bench.restype = C.c_double
+        L.orc_bench.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, DP]
+        L.orc_normals.argtypes = [C.c_uint64, C.c_int, DP]
+        L.orc_mt19937_10000th.restype = C.c_uint32
+        _lib = L
+    return _lib
+
+
+def c2f(a):
+    return np.ascontiguousarray(a, dtype=np.complex128).view(np.float64)
+
+
+class Oracle:
+    """One oracle context over a HamArrays (left trial attached)."""
+
+    def __init__(self, ham, nwalk, walker_rhf, ref_compat=False):
+        self.h = ham
+        self._hs = ham.ham_struct()
+        self._ts = ham.trial_struct()
+        self.nwalk = nwalk
+        self.ptr = lib().orc_create(C.byref(self._hs), C.byref(self._ts),
+                                    int(walker_rhf), int(nwalk), int(ref_compat))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc_destroy(self.ptr)
+            self.ptr = None
+
+    def set_nchol_ene(self, n):
+        lib().orc_set_nchol_ene(self.ptr, int(n))
+
+    def prep_prop(self, ref_up, ref_dn, dt, ene0=0.0, rdm_up=None, rdm_dn=None):
+        ru = np.ascontiguousarray(ref_up, dtype=np.float64)
+        rd = np.ascontiguousarray(ref_dn, dtype=np.float64)
+        lib().orc_prep_prop(self.ptr, dptr(ru), dptr(rd),
+                            dptr(None if rdm_up is None else np.ascontiguousarray(rdm_up)),
+                            dptr(None if rdm_dn is None else np.ascontiguousarray(rdm_dn)),
+                            dt, ene0)
+
+    def prop_data(self):
+        G, N = self.h.nchol, self.h.norb
+        mf = np.zeros(G, np.complex128)
+        e = np.zeros((N, N))
+        pc = np.zeros(1, np.complex128)
+        lib().orc_get_prop_data(self.ptr, dptr(mf.view(np.float64)), dptr(e), dptr(pc.view(np.float64)))
+        return mf, e, pc[0]
+
+    def set_walkers(self, dets, overlaps=None, weights=None):
+        d = c2f(dets)
+        o = None if overlaps is None else c2f(overlaps)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        lib().orc_set_walkers(self.ptr, dptr(d), dptr(o), dptr(w))
+
+    def get_walkers(self):
+        d = np.zeros((self.nwalk, self.h.mat_size), np.complex128)
+        o = np.zeros(self.nwalk, np.complex128)
+        w = np.zeros(self.nwalk)
+        lib().orc_get_walkers(self.ptr, dptr(d.view(np.float64)), dptr(o.view(np.float64)), dptr(w))
+        return d, o, w
+
+    def propagate_walker(self, w, x, eshift, want_vhs=False):
+        N, G = self.h.norb, self.h.nchol
+        v = np.zeros((N, N), np.complex128) if want_vhs else None
+        fb = np.zeros(G, np.complex128)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        f = lib().orc_propagate_walker(self.ptr, int(w), dptr(x), eshift,
+                                       dptr(None if v is None else v.view(np.float64)),
+                                       dptr(fb.view(np.float64)))
+        return (f, fb, v) if want_vhs else (f, fb)
+
+    def propagate(self, draws, eshift, weight_cap):
+        draws = np.ascontiguousarray(draws, dtype=np.float64)
+        f = np.zeros(self.nwalk)
+        nc = C.c_int64(0)
+        lib().orc_propagate(self.ptr, dptr(draws), eshift, weight_cap, dptr(f), C.byref(nc))
+        return f, nc.value
+
+    def orthogonalize(self):
+        lib().orc_orthogonalize(self.ptr)
+
+    def ham_and_overlap(self, w):
+        o = np.zeros(2, np.complex128)
+        lib().orc_ham_and_overlap_walker(self.ptr, int(w), dptr(o.view(np.float64)))
+        return o
+
+    def energy_parts(self, w):
+        a = np.zeros(1, np.complex128); b = np.zeros(1, np.complex128); c = np.zeros(1, np.complex128)
+        lib().orc_energy_parts_walker(self.ptr, int(w), dptr(a.view(np.float64)),
+                                      dptr(b.view(np.float64)), dptr(c.view(np.float64)))
+        return a[0], b[0], c[0]
+
+    def force_bias(self, w):
+        fb = np.zeros(self.h.nchol, np.complex128)
+        lib().orc_force_bias_walker(self.ptr, int(w), dptr(fb.view(np.float64)))
+        return fb
+
+    def overlap(self, w):
+        o = np.zeros(1, np.complex128)
+        lib().orc_overlap_walker(self.ptr, int(w), dptr(o.view(np.float64)))
+        return o[0]
+
+    def green(self, w, slot, n):
+        N = self.h.norb
+        th = np.zeros((N, n), np.complex128)
+        g = np.zeros((N, N), np.complex128)
+        lib().orc_green_walker(self.ptr, int(w), int(slot), dptr(th.view(np.float64)),
+                               dptr(g.view(np.float64)))
+        return th, g
+
+    def ham_and_overlap_dets(self, dets, nchol_ene):
+        dets = np.atleast_2d(np.asarray(dets, dtype=np.complex128))
+        out = np.zeros((dets.shape[0], 2), np.complex128)
+        lib().orc_ham_and_overlap_dets(self.ptr, dptr(c2f(dets)), dets.shape[0], int(nchol_ene),
+                                       dptr(out.view(np.float64)))
+        return out
+
+    def comb(self, zeta):
+        t = np.zeros(1)
+        lib().orc_comb(self.ptr, zeta, dptr(t))
+        return t[0]
+
+    def bench(self, nthreads, nsteps, energy_every, ortho_steps, seed):
+        s = np.zeros(1)
+        r = lib().orc_bench(self.ptr, nthreads, nsteps, energy_every, ortho_steps, seed, dptr(s))
+        return r, s[0]
+
+
+def run_long_prop(case_ham, left_is_rhf, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps,
+                  nsweeps, ortho_steps, nwalk, nranks, walker_rhf, ref_compat=False,
+                  cholesky_threshold=1e-3, burn_iter=0, ene0_guess=1e10, weight_cap=0.0,
+                  max_steps=0):
+    hs = case_ham.ham_struct()
+    ts = case_ham.trial_struct()
+    p = RunParams(seed, dt, nsteps, nsweeps, ortho_steps, burn_iter, nwalk, nranks,
+                  int(walker_rhf), int(ref_compat), cholesky_threshold, ene0_guess,
+                  weight_cap, max_steps)
+    w = np.zeros(nsweeps)
+    e = np.zeros(nsweeps)
+    res = RunResult()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (ref_up, ref_dn, init_up, init_dn)]
+    rc = lib().orc_run_long_prop(C.byref(hs), C.byref(ts), *[dptr(a) for a in arrs], C.byref(p),
+                                 dptr(w), dptr(e), C.byref(res))
+    return rc, w, e, res
+
+
+def blocking(weights, energies):
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    out = np.zeros(2)
+    lib().orc_blocking(dptr(w), dptr(e), len(w), dptr(out))
+    return out

@@ -1,0 +1,300 @@
+"""Pins the CPU oracle (test infrastructure) before anything is compared to it.
+
+1. Golden vectors held by the reference's own tests (tests/DQMC/*/samples.ref):
+   line 1 exactly, and -- in ref_compat mode with 4 virtual ranks seeded like
+   `mpirun -np 4` (executables/DQMC.cpp:58) -- the whole block trajectory.
+2. Standard-library KAT of the RNG stream (std::mt19937 10000th output).
+3. Closed forms / brute force on small synthetic systems: Green's function
+   invariants, force bias and local energy from the full ERI tensor, the
+   one-body propagator against scipy's expm, the Taylor propagation against an
+   independent numpy evaluation, Householder QR against LAPACK (numpy.linalg.qr
+   uses the same real-diagonal convention as Eigen), and the comb against
+   numpy.searchsorted.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle_lib as O
+from vmc_amd.formats import load_case
+from vmc_amd.synthetic import make_system
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CASES = ["rhf_rhf", "uhf_uhf", "rhf_uhf", "uhf_rhf"]
+
+
+def run_case(c, ref_compat, max_steps=0, nsweeps=None):
+    return O.run_long_prop(
+        c.ham, c.ham.trial_rhf, c.ref_up, c.ref_dn, c.init_up, c.init_dn, seed=c.seed, dt=c.dt,
+        nsteps=c.nsteps, nsweeps=nsweeps or c.nsweeps, ortho_steps=c.ortho_steps, nwalk=c.nwalk,
+        nranks=c.nranks, walker_rhf=c.walker_rhf, ref_compat=ref_compat,
+        cholesky_threshold=c.cholesky_threshold, max_steps=max_steps)
+
+
+def test_mt19937_kat():
+    # C++ standard [rand.predef]: 10000th consecutive invocation of a default
+    # constructed std::mt19937 produces 4123659995.
+    assert O.lib().orc_mt19937_10000th() == 4123659995
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("ref_compat", [False, True])
+def test_samples_ref_line1(name, ref_compat):
+    c = load_case(os.path.join(GOLD, name + ".npz"))
+    rc, w, e, res = run_case(c, ref_compat, max_steps=1)
+    assert rc == 0
+    assert w[0] == c.samples[0, 0] == 80.0
+    # samples.ref prints %.10e; the first line is the trial energy (+ delta)
+    assert abs(e[0] - c.samples[0, 1]) < 1e-8 * abs(c.samples[0, 1])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_samples_ref_trajectory(name):
+    """ref_compat + 4 virtual ranks reproduces the reference run block by block
+    (energies within the reference's own test tolerance 1e-7,
+    tests/runDQMC.sh:9, on the first 10 blocks)."""
+    c = load_case(os.path.join(GOLD, name + ".npz"))
+    nb = 10
+    rc, w, e, res = run_case(c, True, nsweeps=c.nsweeps, max_steps=nb * c.nsteps - 1)
+    assert rc == 0
+    assert np.all(np.abs(e[:nb] - c.samples[:nb, 1]) < 1e-7)
+    assert np.all(np.abs(w[:nb] - c.samples[:nb, 0]) < 1e-6 * c.samples[:nb, 0])
+
+
+@pytest.mark.slow
+def test_samples_ref_full_last_line():
+    c = load_case(os.path.join(GOLD, "uhf_rhf.npz"))
+    rc, w, e, res = run_case(c, True)
+    assert rc == 0
+    # the reference's pass criterion: the last line of samples.dat within 1e-7
+    assert abs(e[-1] - c.samples[-1, 1]) < 1e-7
+    assert abs(w[-1] - c.samples[-1, 0]) < 1e-5
+
+
+def test_fp64_mode_statistical_agreement():
+    """Plain FP64 (no float rounding) differs from the reference only at the
+    FP32 level; block energies stay within a few mHa of samples.ref."""
+    c = load_case(os.path.join(GOLD, "uhf_rhf.npz"))
+    rc, w, e, res = run_case(c, False, max_steps=10 * c.nsteps - 1)
+    assert rc == 0
+    assert np.all(np.abs(e[:10] - c.samples[:10, 1]) < 5e-4)
+
+
+# ---------------------------------------------------------------- small systems
+
+def small_system(trial="uhf", na=3, nb=2, N=8, G=12, seed=3):
+    return make_system(N, na, nb, G, seed=seed, trial=trial)
+
+
+def random_walkers(h, nw, rng, scale=0.3):
+    dets = []
+    for _ in range(nw):
+        up = h.phi_up + scale * (rng.standard_normal(h.phi_up.shape) + 1j * rng.standard_normal(h.phi_up.shape))
+        dn = h.phi_dn + scale * (rng.standard_normal(h.phi_dn.shape) + 1j * rng.standard_normal(h.phi_dn.shape))
+        dets.append(h.serial_det(up, dn))
+    return np.array(dets)
+
+
+def eri(h):
+    return np.einsum("gij,gkl->ijkl", h.chol, h.chol)
+
+
+def mixed_energy_bruteforce(h, dets_up_dn, phis, mults):
+    """E = sum_s m_s tr(G_s h) + 1/2 sum (ij|kl)[G_ji G_lk - sum_s m_s G_s,jk G_s,li],
+    G_s = psi_s (Phi_s^T psi_s)^-1 Phi_s^T, from the full ERI tensor."""
+    V = eri(h)
+    e1 = 0
+    Gtot = 0
+    ex = 0
+    O_ = 1
+    for psi, phi, m in zip(dets_up_dn, phis, mults):
+        S = phi.T @ psi
+        O_ *= np.linalg.det(S) ** m
+        Gs = psi @ np.linalg.inv(S) @ phi.T
+        e1 += m * np.sum(Gs * h.h1)
+        Gtot = Gtot + m * Gs
+        ex += m * np.einsum("ijkl,jk,li->", V, Gs, Gs)
+    coul = np.einsum("ijkl,ji,lk->", V, Gtot, Gtot)
+    return h.ecore + e1 + 0.5 * (coul - ex), O_
+
+
+@pytest.mark.parametrize("trial,walker_rhf", [("uhf", False), ("rhf", True), ("rhf", False), ("uhf", True)])
+def test_energy_force_bias_bruteforce(trial, walker_rhf):
+    rng = np.random.default_rng(11)
+    na = nb = 3 if walker_rhf or trial == "rhf" else None
+    h = small_system(trial, na=3, nb=3 if (walker_rhf or trial == "rhf") else 2)
+    nw = 3
+    orc = O.Oracle(h, nw, walker_rhf)
+    dets = random_walkers(h, nw, rng)
+    if walker_rhf:  # RHF walker: psi_dn mirrors psi_up
+        for w in range(nw):
+            up, _ = h.split_det(dets[w])
+            dets[w] = h.serial_det(up, up)
+    orc.set_walkers(dets)
+    for w in range(nw):
+        up, dn = h.split_det(dets[w])
+        if walker_rhf and trial == "rhf":
+            slots = ([up], [h.phi_up], [2])
+        elif walker_rhf:
+            slots = ([up, up], [h.phi_up, h.phi_dn], [1, 1])
+        else:
+            slots = ([up, dn], [h.phi_up, h.phi_dn], [1, 1])
+        E, Ov = mixed_energy_bruteforce(h, *slots)
+        ho = orc.ham_and_overlap(w)
+        assert abs(ho[1] - Ov) < 1e-12 * abs(Ov)
+        assert abs(ho[0] / ho[1] - E) < 1e-11 * abs(E)
+        assert abs(orc.overlap(w) - Ov) < 1e-12 * abs(Ov)
+        # force bias: fb_g = sum_s m_s tr(L_g G_s)
+        fb = np.zeros(h.nchol, complex)
+        for psi, phi, m in zip(*slots):
+            Gs = psi @ np.linalg.inv(phi.T @ psi) @ phi.T
+            fb += m * np.einsum("gij,ji->g", h.chol, Gs)
+        assert np.allclose(orc.force_bias(w), fb, rtol=1e-12, atol=1e-13)
+
+
+def test_green_invariants():
+    rng = np.random.default_rng(5)
+    h = small_system("uhf")
+    orc = O.Oracle(h, 2, False)
+    orc.set_walkers(random_walkers(h, 2, rng))
+    for slot, (phi, n) in enumerate([(h.phi_up, h.nalpha), (h.phi_dn, h.nbeta)]):
+        th, G = orc.green(0, slot, n)
+        assert np.allclose(phi.T @ th, np.eye(n), atol=1e-12)
+        assert np.allclose(G @ G, G, atol=1e-11)
+        assert abs(np.trace(G) - n) < 1e-12
+
+
+def test_trial_overlap_and_background_cancellation():
+    """psi = Phi (orthonormal) => O = 1, and with the trial as reference the
+    force bias equals the mean-field shift (i fb - mf = 0), DQMC/DQMCWalker.cpp:453."""
+    h = small_system("uhf")
+    orc = O.Oracle(h, 1, False)
+    orc.set_walkers(h.serial_det(h.phi_up, h.phi_dn)[None, :])
+    orc.prep_prop(h.phi_up, h.phi_dn, 0.01)
+    mf, E, pc = orc.prop_data()
+    assert abs(orc.overlap(0) - 1.0) < 1e-13
+    fb = orc.force_bias(0)
+    assert np.allclose(1j * fb - mf, 0.0, atol=1e-13)
+
+
+def test_prep_prop_closed_form():
+    h = small_system("uhf", seed=9)
+    dt = 0.01
+    orc = O.Oracle(h, 1, False)
+    orc.prep_prop(h.phi_up, h.phi_dn, dt)
+    mf, E, pc = orc.prop_data()
+    Gu = h.phi_up @ np.linalg.inv(h.phi_up.T @ h.phi_up) @ h.phi_up.T
+    Gd = h.phi_dn @ np.linalg.inv(h.phi_dn.T @ h.phi_dn) @ h.phi_dn.T
+    a = np.einsum("gij,ij->g", h.chol, Gu + Gd)
+    assert np.allclose(mf, 1j * a, atol=1e-13)
+    h1p = h.h1_mod + np.einsum("g,gij->ij", a, h.chol)
+    assert np.allclose(E, scipy.linalg.expm(-dt / 2 * h1p), atol=1e-14)
+    assert abs(pc - (-h.ecore + 0.5 * np.sum(a * a))) < 1e-12
+
+
+@pytest.mark.parametrize("walker_rhf,trial", [(False, "uhf"), (True, "rhf")])
+def test_one_step_independent(walker_rhf, trial):
+    """One propagatePhaselessRG step against an independent numpy statement of
+    A.1 steps 1-10 (fields, VHS, E T5(V) E psi, overlap ratio, phaseless factor)."""
+    rng = np.random.default_rng(7)
+    nb = 3 if trial == "rhf" else 2
+    h = small_system(trial, na=3, nb=nb, seed=4)
+    dt, eshift = 0.01, -1.3
+    orc = O.Oracle(h, 1, walker_rhf)
+    d = random_walkers(h, 1, rng, 0.2)[0]
+    up, dn = h.split_det(d)
+    if walker_rhf:
+        dn = up.copy()
+        d = h.serial_det(up, dn)
+    orc.set_walkers(d[None, :])
+    orc.prep_prop(h.phi_up, h.phi_dn, dt)
+    mf, E, pc = orc.prop_data()
+    x = rng.standard_normal(h.nchol)
+    # independent numpy
+    slots = ([up], [h.phi_up], [2]) if walker_rhf else ([up, dn], [h.phi_up, h.phi_dn], [1, 1])
+    fb = np.zeros(h.nchol, complex)
+    O_old = 1
+    for psi, phi, m in zip(*slots):
+        S = phi.T @ psi
+        O_old *= np.linalg.det(S) ** m
+        fb += m * np.einsum("gij,ji->g", h.chol, psi @ np.linalg.inv(S) @ phi.T)
+    sq = np.sqrt(dt)
+    s = -sq * (1j * fb - mf)
+    fld = x - s
+    shift = np.sum(fld * mf)
+    fbterm = np.sum(x * s - s * s / 2)
+    V = 1j * sq * np.einsum("g,gij->ij", fld, h.chol)
+    def prop(psi):
+        p = E @ psi
+        acc, t = p.copy(), p.copy()
+        for k in range(1, 6):
+            t = V @ t / k
+            acc += t
+        return E @ acc
+    new = [prop(up)] if walker_rhf else [prop(up), prop(dn)]
+    O_new = 1
+    for psi, phi, m in zip(new, slots[1], slots[2]):
+        O_new *= np.linalg.det(phi.T @ psi) ** m
+    imp = np.exp(-sq * shift + fbterm + dt * (eshift + pc)) * O_new / O_old
+    th = np.angle(np.exp(-sq * shift) * O_new / O_old)
+    f_ref = abs(imp) * np.cos(th)
+    if f_ref < 1e-3 or f_ref > 100:
+        f_ref = 0.0
+    f, fb_o, Vo = orc.propagate_walker(0, x, eshift, want_vhs=True)
+    assert np.allclose(fb_o, fb, rtol=1e-12, atol=1e-13)
+    assert np.allclose(Vo, V, rtol=1e-12, atol=1e-14)
+    assert abs(f - f_ref) < 1e-11 * max(1.0, abs(f_ref))
+    dets, ov, _ = orc.get_walkers()
+    u2, d2 = h.split_det(dets[0])
+    assert np.allclose(u2, new[0], rtol=1e-12, atol=1e-13)
+    assert abs(ov[0] - O_new) < 1e-11 * abs(O_new)
+    # truncated Taylor series vs the exact exponential: |diff| <= ||V||^6/6! e^||V||
+    nv = np.linalg.norm(V, 2)
+    import math
+    t5 = sum(np.linalg.matrix_power(V, k) / math.factorial(k) for k in range(6))
+    assert np.linalg.norm(t5 - scipy.linalg.expm(V), 2) <= nv ** 6 / math.factorial(6) * math.exp(nv) * (1 + 1e-9) + 1e-14
+
+
+def test_householder_matches_lapack_gauge():
+    rng = np.random.default_rng(2)
+    h = small_system("uhf")
+    orc = O.Oracle(h, 2, False)
+    dets = random_walkers(h, 2, rng, 0.8)
+    orc.set_walkers(dets)
+    _, ov0, _ = orc.get_walkers()
+    orc.orthogonalize()
+    d2, ov1, _ = orc.get_walkers()
+    for w in range(2):
+        up, dn = h.split_det(dets[w])
+        q_up, r_up = np.linalg.qr(up)
+        q_dn, r_dn = np.linalg.qr(dn)
+        u2, dd2 = h.split_det(d2[w])
+        assert np.allclose(u2, q_up, atol=1e-12)
+        assert np.allclose(dd2, q_dn, atol=1e-12)
+        fac = np.prod(np.diag(r_up)) * np.prod(np.diag(r_dn))
+        assert abs(ov1[w] - ov0[w] / fac) < 1e-11 * abs(ov1[w])
+        # overlap ratio invariance: recomputed overlap of Q equals the stored one
+        assert abs(orc.overlap(w) - ov1[w]) < 1e-11 * abs(ov1[w])
+
+
+def test_comb_matches_searchsorted():
+    rng = np.random.default_rng(8)
+    h = small_system("uhf")
+    nw = 16
+    orc = O.Oracle(h, nw, False)
+    dets = random_walkers(h, nw, rng)
+    wts = rng.uniform(0.0, 2.0, nw)
+    wts[3] = 0.0
+    orc.set_walkers(dets, weights=wts)
+    zeta = 0.37
+    tot = orc.comb(zeta)
+    cum = np.cumsum(wts)
+    idx = np.searchsorted(cum, (np.arange(nw) + zeta) / nw * cum[-1], side="left")
+    d2, _, w2 = orc.get_walkers()
+    assert abs(tot - cum[-1]) < 1e-12
+    assert np.allclose(w2, cum[-1] / nw)
+    assert np.array_equal(d2, dets[idx])
+    assert 3 not in idx

@@ -1,0 +1,83 @@
+"""Readers for the reference's inputs as committed fixtures.
+
+A case mirrors one reference test directory: the FCIDUMP_chol fields
+(utils/integral.cpp:589-661), the rhf.txt / uhf.txt trial text
+(utils/input.cpp:376-383, RHF N x N, UHF N x 2N with alpha columns first),
+the afqmc.json sampling keys, and samples.ref (DQMC/MixedEstimator.cpp:678-683).
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from .abi import HamArrays
+
+
+@dataclass
+class Case:
+    name: str
+    ham: HamArrays          # left trial attached
+    left: str
+    right: str
+    walker_rhf: bool
+    ref_up: np.ndarray      # reference determinant for prepProp / E_ref
+    ref_dn: np.ndarray
+    init_up: np.ndarray     # right-trial sample every walker starts from
+    init_dn: np.ndarray
+    seed: int
+    dt: float
+    nsteps: int
+    nwalk: int
+    nsweeps: int
+    ortho_steps: int
+    cholesky_threshold: float
+    nranks: int
+    samples: np.ndarray     # (nsweeps, 2) weight, energy
+
+
+def walker_is_rhf(nalpha, nbeta, right):
+    """DQMCWalker selection for phaseless runs (executables/DQMC.cpp:74-82,146-154):
+    rhfQ iff nalpha == nbeta and the right state is not UHF."""
+    return nalpha == nbeta and right != "uhf"
+
+
+def trial_orbitals(kind, z, norb, nalpha, nbeta):
+    if kind == "rhf":
+        m = np.asarray(z["rhf"]).reshape(norb, norb)
+        return m[:, :nalpha].copy(), m[:, :nbeta].copy()
+    m = np.asarray(z["uhf"]).reshape(norb, 2 * norb)
+    return m[:, :nalpha].copy(), m[:, norb:norb + nbeta].copy()
+
+
+def load_case(path):
+    z = np.load(path, allow_pickle=False)
+    nelec, norb, ms, nchol = (int(v) for v in z["header"])
+    nalpha = (nelec + ms) // 2
+    nbeta = nelec - nalpha
+    left, right = str(z["left"]), str(z["right"])
+    wrhf = walker_is_rhf(nalpha, nbeta, right)
+    lu, ld = trial_orbitals(left, z, norb, nalpha, nbeta)
+    ham = HamArrays(norb, nalpha, nbeta, float(z["energy_core"][0]),
+                    z["hcore"], z["hcore_mod"], z["chol"], lu, ld,
+                    trial_rhf=(left == "rhf"))
+    # reference dets: rhf.txt for an RHF walker, uhf.txt otherwise
+    # (DQMC/MixedEstimator.cpp:338-351)
+    ru, rd = trial_orbitals("rhf" if wrhf else "uhf", z, norb, nalpha, nbeta)
+    iu, idn = trial_orbitals(right, z, norb, nalpha, nbeta)
+    name = path.split("/")[-1].replace(".npz", "")
+    return Case(name, ham, left, right, wrhf, ru, rd, iu, idn, int(z["seed"]),
+                float(z["dt"]), int(z["nsteps"]), int(z["nwalk"]),
+                int(z["stochasticIter"]), int(z["orthoSteps"]),
+                float(z["choleskyThreshold"]), int(z["nranks"]),
+                np.asarray(z["samples"]))
+
+
+def read_text_matrix(path, rows, cols):
+    """readMat (utils/input.cpp:376-383): whitespace text, row-major."""
+    return np.loadtxt(path).reshape(rows, cols)
+
+
+def write_samples(path, weights, energies):
+    """samples.dat writer, DQMC/MixedEstimator.cpp:678-683 ("%.7e      %.10e ")."""
+    with open(path, "w") as f:
+        for w, e in zip(weights, energies):
+            f.write("%.7e      %.10e \n" % (w, e))
