@@ -182,6 +182,10 @@ int afq_comb(afq_ctx* ctx, double zeta, double* total_weight);
  * force_bias, fields, vhs, propagate, green, weight, ortho, energy, comb. */
 int afq_stage_times(afq_ctx* ctx, double* ms9, int32_t reset);
 
+/* Parity hook for the FP64 MFMA tile engine every contraction above uses:
+ * C = A B for host row-major A (M x K), B (K x N); C (M x N). */
+int afq_debug_gemm_f64(const double* A, const double* B, double* C, int32_t M, int32_t N, int32_t K);
+
 /* Synchronise the context's stream. */
 int afq_sync(afq_ctx* ctx);
 

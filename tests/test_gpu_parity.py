@@ -1,0 +1,235 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Plain FP64 mode must reproduce the oracle's per-step walker orbitals, overlaps,
+weights, force biases and local energies to 1e-10 relative when both are fed
+the same auxiliary-field draws.  Shapes cover the four reference fixtures
+(tests/DQMC/{rhf_rhf,uhf_uhf,rhf_uhf,uhf_rhf}) and synthetic systems that cross
+the kernels' tile boundaries (n up to 60, N > 64, nalpha != nbeta).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from vmc_amd.afqmc import Context, debug_gemm_f64
+from vmc_amd.formats import load_case
+from vmc_amd.synthetic import make_system
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-10
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / den)
+
+
+def rel_each(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def test_mfma_engine_layout():
+    rng = np.random.default_rng(0)
+    for M, K, N in [(16, 4, 16), (70, 45, 150), (130, 300, 257)]:
+        A = rng.standard_normal((M, K))
+        B = rng.standard_normal((K, N))
+        Cg = debug_gemm_f64(A, B)
+        assert rel(Cg, A @ B) < 1e-13, (M, K, N)
+
+
+def perturbed_walkers(h, W, walker_rhf, rng, scale=0.25):
+    dets = []
+    for _ in range(W):
+        up = h.phi_up + scale * (rng.standard_normal(h.phi_up.shape) + 1j * rng.standard_normal(h.phi_up.shape))
+        if walker_rhf:
+            dn = up.copy()
+        else:
+            dn = h.phi_dn + scale * (rng.standard_normal(h.phi_dn.shape) + 1j * rng.standard_normal(h.phi_dn.shape))
+        dets.append(h.serial_det(up, dn))
+    return np.array(dets)
+
+
+SYSTEMS = {
+    # name: (builder, walker_rhf)
+    "rhf_rhf": (lambda: load_case(os.path.join(GOLD, "rhf_rhf.npz")), None),
+    "uhf_uhf": (lambda: load_case(os.path.join(GOLD, "uhf_uhf.npz")), None),
+    "rhf_uhf": (lambda: load_case(os.path.join(GOLD, "rhf_uhf.npz")), None),
+    "uhf_rhf": (lambda: load_case(os.path.join(GOLD, "uhf_rhf.npz")), None),
+    "c0_rhf": (lambda: make_system(10, 5, 5, 40, seed=101, trial="rhf"), True),
+    "uhf_33_31": (lambda: make_system(70, 33, 31, 130, seed=102, trial="uhf"), False),
+    "uhf_n60": (lambda: make_system(150, 60, 60, 100, seed=103, trial="uhf"), False),
+    "c1_small": (lambda: make_system(100, 20, 20, 400, seed=104, trial="uhf"), False),
+}
+
+
+def build(name, W, ref_compat=False, dt=0.005):
+    obj, wrhf = SYSTEMS[name]
+    obj = obj()
+    if hasattr(obj, "ham"):  # reference fixture
+        h, wrhf, ru, rd = obj.ham, obj.walker_rhf, obj.ref_up, obj.ref_dn
+    else:
+        h, ru, rd = obj, obj.phi_up, obj.phi_dn
+    ctx = Context(h, W, wrhf, ref_compat=ref_compat)
+    orc = O.Oracle(h, W, wrhf, ref_compat=ref_compat)
+    ctx.prep_prop(ru, rd, dt)
+    orc.prep_prop(ru, rd, dt)
+    return h, wrhf, ctx, orc
+
+
+def check_state(h, ctx, orc, tol=TOL, what=""):
+    dg, og, wg = ctx.get_walkers()
+    do, oo, wo = orc.get_walkers()
+    assert rel(dg, do) < tol, (what, "dets", rel(dg, do))
+    live = wo > 0
+    if live.any():
+        assert rel_each(og[live], oo[live]) < tol, (what, "overlaps", rel_each(og[live], oo[live]))
+    assert rel(wg, wo) < tol, (what, "weights", rel(wg, wo))
+    return dg, og, wg
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_green_force_bias_energy(name):
+    W = 5
+    h, wrhf, ctx, orc = build(name, W)
+    rng = np.random.default_rng(1)
+    dets = perturbed_walkers(h, W, wrhf, rng)
+    ctx.set_walkers(dets)
+    orc.set_walkers(dets)
+    check_state(h, ctx, orc, what="set")
+    fbg = ctx.force_bias()
+    for w in range(W):
+        assert rel(fbg[w], orc.force_bias(w)) < TOL, ("fb", w)
+    e, ho = ctx.measure()
+    for w in range(W):
+        o = orc.ham_and_overlap(w)
+        assert rel_each(ho[w], o) < TOL, ("hamAndOverlap", w, ho[w], o)
+        assert abs(e[w] - (o[0] / o[1]).real) < TOL * abs(e[w]), ("eloc", w)
+
+
+@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_uhf", "uhf_33_31", "c1_small"])
+@pytest.mark.parametrize("nchol_ene_frac", [0.5, 1.0])
+def test_ham_and_overlap_dets(name, nchol_ene_frac):
+    h, wrhf, ctx, orc = build(name, 2)
+    rng = np.random.default_rng(4)
+    dets = perturbed_walkers(h, 3, wrhf, rng)
+    ge = int(nchol_ene_frac * h.nchol)
+    a = ctx.ham_and_overlap(dets, ge)
+    b = orc.ham_and_overlap_dets(dets, ge)
+    assert rel_each(a, b) < TOL
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_trajectory(name):
+    """Replay the same draws through both sides: steps with QR every 3 steps,
+    energy measurement and comb every 4, one dead walker and one capped walker."""
+    W = 6
+    h, wrhf, ctx, orc = build(name, W)
+    rng = np.random.default_rng(2)
+    dets = perturbed_walkers(h, W, wrhf, rng, 0.1)
+    weights = np.ones(W)
+    weights[1] = 0.0          # dead: must not move or draw
+    weights[2] = 5e-9         # below the 1e-8 liveness threshold
+    ctx.set_walkers(dets, weights=weights)
+    orc.set_walkers(dets, weights=weights)
+    cap = 100.0
+    eshift = -1.0
+    nsteps = 8 if h.norb < 120 else 5
+    for step in range(1, nsteps + 1):
+        x = rng.standard_normal((W, h.nchol))
+        fg, ncg = ctx.propagate(x, eshift)
+        fo, nco = orc.propagate(x, eshift, cap)
+        assert rel(fg, fo) < TOL, (step, fg, fo)
+        assert ncg == nco
+        check_state(h, ctx, orc, what="step %d" % step)
+        if step % 3 == 0:
+            ctx.orthogonalize()
+            orc.orthogonalize()
+            check_state(h, ctx, orc, what="qr %d" % step)
+        if step % 4 == 0:
+            e, ho = ctx.measure()
+            _, _, wo = orc.get_walkers()
+            for w in range(W):
+                if wo[w] != 0:
+                    o = orc.ham_and_overlap(w)
+                    assert abs(e[w] - (o[0] / o[1]).real) < TOL * abs(e[w]), (step, w)
+            zeta = float(rng.uniform())
+            tg = ctx.comb(zeta)
+            to = orc.comb(zeta)
+            assert abs(tg - to) < TOL * abs(to)
+            check_state(h, ctx, orc, what="comb %d" % step)
+        # force bias of the new state (used by the next step) matches too
+        fbg = ctx.force_bias()
+        _, _, wo = orc.get_walkers()
+        for w in range(W):
+            if wo[w] > 1e-8:
+                assert rel(fbg[w], orc.force_bias(w)) < TOL, ("fb", step, w)
+
+
+@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_rhf", "uhf_33_31"])
+def test_trajectory_ref_compat(name):
+    """ref_compat rounds the VHS to float on both sides; the GPU forms the sum in
+    a different order, so rare one-ulp float differences are allowed (1e-7)."""
+    W = 4
+    h, wrhf, ctx, orc = build(name, W, ref_compat=True)
+    rng = np.random.default_rng(3)
+    dets = perturbed_walkers(h, W, wrhf, rng, 0.1)
+    ctx.set_walkers(dets)
+    orc.set_walkers(dets)
+    for step in range(1, 7):
+        x = rng.standard_normal((W, h.nchol))
+        fg, _ = ctx.propagate(x, -0.5)
+        fo, _ = orc.propagate(x, -0.5, 100.0)
+        assert rel(fg, fo) < 1e-7
+        check_state(h, ctx, orc, tol=1e-7, what="compat step %d" % step)
+
+
+def test_weight_guards():
+    """Factor zeroing (<1e-3, >100, NaN) and the weight cap, DQMC/DQMCWalker.cpp:501 and
+    DQMC/MixedEstimator.cpp:463-466, forced from the host through eshift."""
+    W = 4
+    h, wrhf, ctx, orc = build("uhf_uhf", W)
+    rng = np.random.default_rng(5)
+    dets = perturbed_walkers(h, W, wrhf, rng, 0.05)
+    ctx.set_walkers(dets)
+    orc.set_walkers(dets)
+    x = rng.standard_normal((W, h.nchol))
+    # eshift far below: every factor < 1e-3 -> 0
+    fg, _ = ctx.propagate(x, eshift=-5000.0)
+    assert np.all(fg == 0.0)
+    assert np.all(ctx.weights() == 0.0)
+    # fresh population, eshift far above: factor > 100 -> 0
+    ctx.set_walkers(dets)
+    fg, _ = ctx.propagate(x, eshift=+5000.0)
+    assert np.all(fg == 0.0)
+    # weight cap: start at 99.9 with a factor slightly above 1
+    ctx.set_walkers(dets, weights=np.full(W, 99.99))
+    orc.set_walkers(dets, weights=np.full(W, 99.99))
+    fg, ncg = ctx.propagate(x, eshift=50.0)
+    fo, nco = orc.propagate(x, 50.0, 100.0)
+    assert ncg == nco
+    assert rel(ctx.weights(), orc.get_walkers()[2]) < TOL
+
+
+def test_device_rng_stream():
+    """draws=None: counter-based device normals; same (seed, step) reproduces,
+    a different step does not."""
+    W = 8
+    h, wrhf, ctx, _ = build("c0_rhf", W)
+    rng = np.random.default_rng(6)
+    dets = perturbed_walkers(h, W, wrhf, rng, 0.1)
+    ctx.set_walkers(dets)
+    f1, _ = ctx.propagate(None, -1.0, seed=7, step=3)
+    d1, _, _ = ctx.get_walkers()
+    ctx.set_walkers(dets)
+    f2, _ = ctx.propagate(None, -1.0, seed=7, step=3)
+    d2, _, _ = ctx.get_walkers()
+    assert np.array_equal(d1, d2) and np.array_equal(f1, f2)
+    ctx.set_walkers(dets)
+    ctx.propagate(None, -1.0, seed=7, step=4)
+    d3, _, _ = ctx.get_walkers()
+    assert not np.allclose(d1, d3)

@@ -1,0 +1,69 @@
+"""Builds the HIP library vmc_amd/libvmc_afqmc.so for gfx950 (in-tree).
+
+    python -m vmc_amd.build            # compile changed sources, link
+    python -m vmc_amd.build --force    # recompile everything
+
+Every HIP source is compiled with hipcc --offload-arch=gfx950 and with packed
+FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+import argparse
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libvmc_afqmc.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+SOURCES = ["afq_kernels.hip", "afq_ctx.cpp", "afq_driver.cpp"]
+HEADERS = ["afq_device.hpp", "afq_internal.hpp", "afq_driver.hpp"]
+FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-I" + os.path.join(ROOT, "include")]
+
+
+def _deps_mtime():
+    paths = [os.path.join(CSRC, h) for h in HEADERS if os.path.exists(os.path.join(CSRC, h))]
+    paths.append(os.path.join(ROOT, "include", "vmc_afqmc.h"))
+    return max(os.path.getmtime(p) for p in paths)
+
+
+def build(force=False, verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    hdr_t = _deps_mtime()
+    for src in SOURCES:
+        sp = os.path.join(CSRC, src)
+        if not os.path.exists(sp):
+            continue
+        op = os.path.join(OBJ, src + ".o")
+        objs.append(op)
+        if (not force and os.path.exists(op) and os.path.getmtime(op) > os.path.getmtime(sp)
+                and os.path.getmtime(op) > hdr_t):
+            continue
+        cmd = [HIPCC] + FLAGS + (["-x", "hip"] if src.endswith(".cpp") else []) + ["-c", sp, "-o", op]
+        if verbose:
+            print("[build]", " ".join(os.path.basename(x) if x.startswith(CSRC) else x for x in cmd),
+                  file=sys.stderr)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc failed for %s:\n%s" % (src, err))
+        if err.strip() and verbose:
+            print(err, file=sys.stderr)
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stderr)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    a = ap.parse_args()
+    print(build(force=a.force))

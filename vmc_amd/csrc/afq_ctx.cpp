@@ -1,0 +1,682 @@
+// Host side of the C ABI (include/vmc_afqmc.h): the device context that holds
+// one rank's walker population, its setup math (mean-field shifts, one-body
+// propagator) and the stream-ordered step.  The step's arithmetic runs in the
+// HIP kernels of afq_kernels.hip; nothing here falls back to the CPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "afq_internal.hpp"
+
+using afq::DevState;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevError : std::runtime_error {
+  int code;
+  DevError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw DevError(AFQ_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+void launch_check(const char* what) { hip_check(hipGetLastError(), what); }
+
+template <class T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+  return static_cast<T*>(p);
+}
+
+// exp(s*H) for a real symmetric H by scaling and squaring of the Taylor series
+// (the reference uses Eigen's Pade MatrixBase::exp, DQMC/DQMCWalker.cpp:102).
+std::vector<double> expm_scaled(const std::vector<double>& H, int N, double s) {
+  double nrm = 0.0;
+  for (int i = 0; i < N; ++i) {
+    double r = 0.0;
+    for (int j = 0; j < N; ++j) r += std::fabs(H[(size_t)i * N + j]);
+    nrm = std::max(nrm, r);
+  }
+  nrm *= std::fabs(s);
+  int sq = 0;
+  while (nrm > 0.125) { nrm *= 0.5; ++sq; }
+  const double a = s / std::ldexp(1.0, sq);
+  std::vector<double> X((size_t)N * N), term((size_t)N * N), tmp((size_t)N * N), E((size_t)N * N, 0.0);
+  for (size_t q = 0; q < X.size(); ++q) X[q] = a * H[q];
+  for (int i = 0; i < N; ++i) E[(size_t)i * N + i] = 1.0;
+  term = E;
+  auto matmul = [&](const std::vector<double>& A, const std::vector<double>& B, std::vector<double>& C) {
+    std::fill(C.begin(), C.end(), 0.0);
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < N; ++k) {
+        const double aik = A[(size_t)i * N + k];
+        if (aik == 0.0) continue;
+        const double* b = &B[(size_t)k * N];
+        double* c = &C[(size_t)i * N];
+        for (int j = 0; j < N; ++j) c[j] += aik * b[j];
+      }
+  };
+  for (int k = 1; k <= 18; ++k) {  // ||X|| <= 1/8: 18 terms reach far below double rounding
+    matmul(term, X, tmp);
+    for (size_t q = 0; q < tmp.size(); ++q) { term[q] = tmp[q] / k; E[q] += term[q]; }
+  }
+  for (int k = 0; k < sq; ++k) {
+    matmul(E, E, tmp);
+    E.swap(tmp);
+  }
+  return E;
+}
+
+}  // namespace
+
+struct afq_ctx {
+  DevState s;
+  afq_params params{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool prepped = false;
+  bool have_walkers = false;
+  std::vector<double> h1, h1mod, chol;  // host copies (setup math)
+  std::vector<double> phi[2];
+  std::vector<float> dummy;
+  // stage timing
+  struct EvPair { hipEvent_t a, b; int stage; };
+  std::vector<EvPair> pending, pool;
+  double stage_ms[afq::ST_COUNT] = {0};
+  bool timing = true;
+
+  hipEvent_t ev_get() {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  void stage_begin(int st, EvPair& p) {
+    if (!timing) return;
+    if (!pool.empty()) { p = pool.back(); pool.pop_back(); }
+    else { p.a = ev_get(); p.b = ev_get(); }
+    p.stage = st;
+    hip_check(hipEventRecord(p.a, stream), "hipEventRecord");
+  }
+  void stage_end(EvPair& p) {
+    if (!timing) return;
+    hip_check(hipEventRecord(p.b, stream), "hipEventRecord");
+    pending.push_back(p);
+    if (pending.size() > 4096) resolve();
+  }
+  void resolve() {
+    for (auto& p : pending) {
+      hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      stage_ms[p.stage] += ms;
+      pool.push_back(p);
+    }
+    pending.clear();
+  }
+};
+
+namespace {
+
+int fail(afq_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  else g_create_error = m;
+  return code;
+}
+
+#define AFQ_GUARD_BEGIN try {
+#define AFQ_GUARD_END(ctx)                                      \
+  }                                                             \
+  catch (const DevError& e) { return fail(ctx, e.code, e.what()); } \
+  catch (const std::exception& e) { return fail(ctx, AFQ_ERR_ARG, e.what()); }
+
+struct Stage {
+  afq_ctx* c;
+  afq_ctx::EvPair p;
+  Stage(afq_ctx* c_, int st) : c(c_) { c->stage_begin(st, p); }
+  ~Stage() { try { c->stage_end(p); } catch (...) {} }
+};
+
+void free_all(DevState& s) {
+  double* ptrs[] = {s.E, s.Lp, s.R, s.phi[0], s.phi[1], s.hphi[0], s.hphi[1], s.mf, s.psi, s.theta, s.fb,
+                    s.logO, s.e1, s.logdet, s.weights, s.factors, s.psi2, s.theta2, s.fb2, s.logO2, s.e12,
+                    s.fields, s.sft, s.P, s.t0, s.t1, s.accb, s.draws, s.Xp, s.eloc, s.ho, s.cum};
+  for (double* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (s.idx) (void)hipFree(s.idx);
+  if (s.ncap) (void)hipFree(s.ncap);
+}
+
+// Walker-dependent buffers for a population of W.
+void alloc_walkers(DevState& s) {
+  const int64_t W = s.W;
+  s.psi = dalloc<double>(2 * W * s.ms);
+  s.psi2 = dalloc<double>(2 * W * s.ms);
+  s.theta = dalloc<double>(2 * W * s.Ktot);
+  s.theta2 = dalloc<double>(2 * W * s.Ktot);
+  s.fb = dalloc<double>(2 * W * s.G);
+  s.fb2 = dalloc<double>(2 * W * s.G);
+  s.logO = dalloc<double>(2 * W);
+  s.logO2 = dalloc<double>(2 * W);
+  s.e1 = dalloc<double>(2 * W * s.nslots);
+  s.e12 = dalloc<double>(2 * W * s.nslots);
+  s.logdet = dalloc<double>(2 * W * s.nslots);
+  s.weights = dalloc<double>(W);
+  s.factors = dalloc<double>(W);
+  s.fields = dalloc<double>(2 * W * s.G);
+  s.sft = dalloc<double>(4 * W);
+  s.P = dalloc<double>(2 * W * s.T);
+  s.t0 = dalloc<double>(2 * W * s.ms);
+  s.t1 = dalloc<double>(2 * W * s.ms);
+  s.accb = dalloc<double>(2 * W * s.ms);
+  s.draws = dalloc<double>(W * s.G);
+  s.Xp = dalloc<double>(2 * (int64_t)s.nslots * W * s.ngt_cap);
+  s.eloc = dalloc<double>(W);
+  s.ho = dalloc<double>(4 * W);
+  s.cum = dalloc<double>(W);
+  s.idx = dalloc<int>(W);
+  s.ncap = dalloc<unsigned long long>(1);
+}
+
+void upload(double* d, const double* h, size_t n, hipStream_t st) {
+  hip_check(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
+}
+void download(double* h, const double* d, size_t n, hipStream_t st) {
+  hip_check(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+// Mean-field shifts and propagator constant of DQMCWalker::prepPropR
+// (DQMC/DQMCWalker.cpp:32-103), phaseless branch; returns h1' = h1Mod - sum mf_g (i L_g).
+void prep_math(afq_ctx* c, const double* gref_up, const double* gref_dn, double dt,
+               std::vector<double>& mf, std::vector<double>& h1p, double& pc) {
+  DevState& s = c->s;
+  const int N = s.N, G = s.G;
+  const size_t NN = (size_t)N * N;
+  h1p = c->h1mod;
+  mf.assign(2 * (size_t)G, 0.0);
+  double constant = -s.ecore;  // ene0 cancels in the phaseless constant
+  for (int g = 0; g < G; ++g) {
+    const double* L = &c->chol[g * NN];
+    double a = 0.0;
+    for (size_t q = 0; q < NN; ++q) {
+      const double l = s.ref_compat ? (double)(float)L[q] : L[q];
+      a += (gref_up[q] + gref_dn[q]) * l;
+    }
+    // mf = i a;  constant -= mf^2/2 = +a^2/2;  h1' -= mf * (i L) = + a L
+    mf[2 * g] = 0.0;
+    mf[2 * g + 1] = a;
+    constant += 0.5 * a * a;
+    for (size_t q = 0; q < NN; ++q) {
+      const double l = s.ref_compat ? (double)(float)L[q] : L[q];
+      h1p[q] += a * l;
+    }
+  }
+  pc = constant;
+  (void)dt;
+}
+
+void green_ref(const double* ref, int N, int n, std::vector<double>& G) {
+  // G = ref (ref^T ref)^-1 ref^T by Gauss-Jordan on the n x n Gram matrix
+  std::vector<double> S((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double a = 0.0;
+      for (int k = 0; k < N; ++k) a += ref[(size_t)k * n + i] * ref[(size_t)k * n + j];
+      S[(size_t)i * n + j] = a;
+    }
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(S[(size_t)i * n + k]) > std::fabs(S[(size_t)p * n + k])) p = i;
+    for (int j = 0; j < n; ++j) {
+      std::swap(S[(size_t)k * n + j], S[(size_t)p * n + j]);
+      std::swap(inv[(size_t)k * n + j], inv[(size_t)p * n + j]);
+    }
+    const double d = S[(size_t)k * n + k];
+    for (int j = 0; j < n; ++j) { S[(size_t)k * n + j] /= d; inv[(size_t)k * n + j] /= d; }
+    for (int i = 0; i < n; ++i) {
+      if (i == k) continue;
+      const double f = S[(size_t)i * n + k];
+      for (int j = 0; j < n; ++j) {
+        S[(size_t)i * n + j] -= f * S[(size_t)k * n + j];
+        inv[(size_t)i * n + j] -= f * inv[(size_t)k * n + j];
+      }
+    }
+  }
+  G.assign((size_t)N * N, 0.0);
+  std::vector<double> tmp((size_t)N * n, 0.0);  // ref * inv
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < n; ++j) {
+      double a = 0.0;
+      for (int k = 0; k < n; ++k) a += ref[(size_t)i * n + k] * inv[(size_t)k * n + j];
+      tmp[(size_t)i * n + j] = a;
+    }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      double a = 0.0;
+      for (int k = 0; k < n; ++k) a += tmp[(size_t)i * n + k] * ref[(size_t)j * n + k];
+      G[(size_t)i * N + j] = a;
+    }
+}
+
+// Green's function, log overlap and force bias of the current population.
+void refresh_walkers(afq_ctx* c, bool overlaps_from_dets) {
+  DevState& s = c->s;
+  {
+    Stage t(c, afq::ST_GREEN);
+    afq::launch_green(s, true, c->stream);
+    launch_check("green");
+  }
+  if (overlaps_from_dets) {
+    afq::launch_logO_from_logdet(s, c->stream);
+    launch_check("logO");
+  }
+  Stage t(c, afq::ST_FB);
+  afq::launch_force_bias(s, c->stream);
+  launch_check("force_bias");
+}
+
+}  // namespace
+
+extern "C" {
+
+int afq_version(void) { return AFQ_ABI_VERSION; }
+
+const char* afq_last_error(const afq_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* params, afq_ctx** out) {
+  if (!ham || !left || !params || !out) return fail(nullptr, AFQ_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int N = ham->norb, na = ham->nalpha, nb = ham->nbeta, G = ham->nchol, W = params->nwalk;
+  if (N <= 0 || na <= 0 || nb < 0 || G <= 0 || W <= 0 || na > N || nb > N)
+    return fail(nullptr, AFQ_ERR_ARG, "bad sizes");
+  if (!ham->h1 || !ham->h1_mod || !ham->chol || !left->phi_up || (!left->rhf && !left->phi_dn))
+    return fail(nullptr, AFQ_ERR_ARG, "null Hamiltonian / trial pointer");
+  if ((params->walker_rhf || left->rhf) && na != nb)
+    return fail(nullptr, AFQ_ERR_ARG, "RHF walker / RHF trial needs nalpha == nbeta");
+  if (nb == 0) return fail(nullptr, AFQ_ERR_UNSUPPORTED, "nbeta == 0 not supported");
+  if (na > 64 || nb > 64)
+    return fail(nullptr, AFQ_ERR_UNSUPPORTED, "more than 64 electrons per spin not supported by this build");
+  if (params->prec_fb != AFQ_PREC_FP64 || params->prec_vhs != AFQ_PREC_FP64 ||
+      params->prec_exchange != AFQ_PREC_FP64)
+    return fail(nullptr, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
+  afq_ctx* c = new afq_ctx();
+  try {
+    c->params = *params;
+    hip_check(hipSetDevice(params->device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    DevState& s = c->s;
+    s.N = N; s.na = na; s.nb = nb; s.G = G; s.GE = G; s.W = W;
+    s.T = (int64_t)N * (N + 1) / 2;
+    s.ms = (int64_t)N * (na + nb);
+    s.walker_rhf = params->walker_rhf != 0;
+    s.trial_rhf = left->rhf != 0;
+    s.ref_compat = params->ref_compat != 0;
+    s.ecore = ham->ecore;
+    s.weight_cap = params->weight_cap > 0 ? params->weight_cap : std::max(100.0, W / 10.0);
+    s.det_n[0] = na; s.det_n[1] = nb;
+    s.det_off[0] = 0; s.det_off[1] = (int64_t)N * na;
+    s.ndets_prop = s.walker_rhf ? 1 : 2;
+    if (s.walker_rhf && s.trial_rhf) {
+      s.nslots = 1;
+      s.slot[0] = afq::Slot{na, 0, 0, 2, 0};
+      s.slot[1] = s.slot[0];
+    } else {
+      s.nslots = 2;
+      s.slot[0] = afq::Slot{na, 0, 0, 1, 0};
+      s.slot[1] = afq::Slot{nb, s.walker_rhf ? 0 : 1, s.trial_rhf ? 0 : 1, 1, (int64_t)N * na};
+    }
+    s.Ktot = 0;
+    for (int q = 0; q < s.nslots; ++q) s.Ktot += (int64_t)N * s.slot[q].n;
+    int ngt = 0;
+    for (int q = 0; q < s.nslots; ++q)
+      ngt = std::max(ngt, (G + afq::exchange_tile_g(s.slot[q].n) - 1) / afq::exchange_tile_g(s.slot[q].n));
+    s.ngt_cap = ngt;
+    // host copies for setup math
+    const size_t NN = (size_t)N * N;
+    c->h1.assign(ham->h1, ham->h1 + NN);
+    c->h1mod.assign(ham->h1_mod, ham->h1_mod + NN);
+    c->chol.assign(ham->chol, ham->chol + (size_t)G * NN);
+    c->phi[0].assign(left->phi_up, left->phi_up + (size_t)N * na);
+    c->phi[1] = left->rhf ? c->phi[0] : std::vector<double>(left->phi_dn, left->phi_dn + (size_t)N * nb);
+    // device Hamiltonian / trial
+    double* Ld = dalloc<double>((size_t)G * NN);
+    upload(Ld, c->chol.data(), (size_t)G * NN, c->stream);
+    s.Lp = dalloc<double>((size_t)G * s.T);
+    afq::launch_pack_chol(Ld, s.Lp, N, G, s.T, s.ref_compat, c->stream);
+    launch_check("pack_chol");
+    s.R = dalloc<double>((size_t)G * s.Ktot);
+    for (int p = 0; p < 2; ++p) {
+      const int n = p == 0 ? na : nb;
+      s.phi[p] = dalloc<double>((size_t)N * n);
+      upload(s.phi[p], c->phi[p].data(), (size_t)N * n, c->stream);
+      std::vector<double> hp((size_t)N * n, 0.0);
+      for (int i = 0; i < N; ++i)
+        for (int k = 0; k < N; ++k) {
+          const double h = c->h1[(size_t)i * N + k];
+          for (int j = 0; j < n; ++j) hp[(size_t)i * n + j] += h * c->phi[p][(size_t)k * n + j];
+        }
+      s.hphi[p] = dalloc<double>((size_t)N * n);
+      upload(s.hphi[p], hp.data(), hp.size(), c->stream);
+    }
+    for (int q = 0; q < s.nslots; ++q) {
+      const afq::Slot& sl = s.slot[q];
+      afq::launch_rotate_chol(Ld, s.phi[sl.phi], s.R, N, sl.n, G, s.Ktot, sl.koff, c->stream);
+      launch_check("rotate_chol");
+    }
+    hip_check(hipStreamSynchronize(c->stream), "setup sync");
+    hip_check(hipFree(Ld), "hipFree");
+    s.E = dalloc<double>(NN);
+    s.mf = dalloc<double>(2 * (size_t)G);
+    alloc_walkers(s);
+    afq::launch_fill(s.weights, W, 0.0, c->stream);
+    hip_check(hipStreamSynchronize(c->stream), "setup sync");
+  } catch (const DevError& e) {
+    free_all(c->s);
+    delete c;
+    return fail(nullptr, e.code, e.what());
+  } catch (const std::exception& e) {
+    free_all(c->s);
+    delete c;
+    return fail(nullptr, AFQ_ERR_DEVICE, e.what());
+  }
+  *out = c;
+  return AFQ_OK;
+}
+
+int afq_destroy(afq_ctx* c) {
+  if (!c) return AFQ_OK;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto& p : c->pool) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  free_all(c->s);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AFQ_OK;
+}
+
+int afq_set_nchol_ene(afq_ctx* c, int32_t n) {
+  if (!c) return AFQ_ERR_ARG;
+  if (n < 0 || n > c->s.G) return fail(c, AFQ_ERR_ARG, "nchol_ene out of range");
+  c->s.GE = n;
+  return AFQ_OK;
+}
+
+int afq_prep_prop(afq_ctx* c, const double* ref_up, const double* ref_dn, const double* rdm_up,
+                  const double* rdm_dn, double dt, double ene0) {
+  if (!c) return AFQ_ERR_ARG;
+  (void)ene0;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  if (!(dt > 0.0)) return fail(c, AFQ_ERR_ARG, "dt must be positive");
+  std::vector<double> gu, gd;
+  if (rdm_up && rdm_dn) {
+    gu.assign(rdm_up, rdm_up + (size_t)s.N * s.N);
+    gd.assign(rdm_dn, rdm_dn + (size_t)s.N * s.N);
+  } else {
+    if (!ref_up || !ref_dn) return fail(c, AFQ_ERR_ARG, "reference determinant required");
+    green_ref(ref_up, s.N, s.na, gu);
+    green_ref(ref_dn, s.N, s.nb, gd);
+  }
+  std::vector<double> mf, h1p;
+  double pc = 0.0;
+  prep_math(c, gu.data(), gd.data(), dt, mf, h1p, pc);
+  const std::vector<double> E = expm_scaled(h1p, s.N, -dt / 2.0);
+  s.dt = dt;
+  s.prop_const_re = pc;
+  s.prop_const_im = 0.0;
+  upload(s.mf, mf.data(), mf.size(), c->stream);
+  upload(s.E, E.data(), E.size(), c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "prep sync");
+  c->prepped = true;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_set_walkers(afq_ctx* c, const double* dets, const double* overlaps, const double* weights) {
+  if (!c || !dets) return c ? fail(c, AFQ_ERR_ARG, "null dets") : AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  upload(s.psi, dets, 2 * (size_t)s.W * s.ms, c->stream);
+  if (weights) upload(s.weights, weights, s.W, c->stream);
+  else afq::launch_fill(s.weights, s.W, 1.0, c->stream);
+  if (overlaps) {
+    std::vector<double> lo(2 * (size_t)s.W);
+    for (int w = 0; w < s.W; ++w) {
+      const std::complex<double> o(overlaps[2 * w], overlaps[2 * w + 1]);
+      lo[2 * w] = std::log(std::abs(o));
+      lo[2 * w + 1] = std::arg(o);
+    }
+    upload(s.logO, lo.data(), lo.size(), c->stream);
+  }
+  refresh_walkers(c, overlaps == nullptr);
+  hip_check(hipStreamSynchronize(c->stream), "set_walkers sync");
+  c->have_walkers = true;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_get_walkers(afq_ctx* c, double* dets, double* overlaps, double* weights) {
+  if (!c) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  if (dets) {
+    download(dets, s.psi, 2 * (size_t)s.W * s.ms, c->stream);
+    if (s.walker_rhf) {  // det[1] mirrors det[0] for an RHF walker (DQMC/DQMCWalker.cpp:486)
+      const size_t n0 = 2 * (size_t)s.N * s.na;
+      for (int w = 0; w < s.W; ++w)
+        std::memcpy(dets + 2 * (size_t)w * s.ms + 2 * s.det_off[1], dets + 2 * (size_t)w * s.ms, n0 * sizeof(double));
+    }
+  }
+  if (overlaps) {
+    std::vector<double> lo(2 * (size_t)s.W);
+    download(lo.data(), s.logO, lo.size(), c->stream);
+    for (int w = 0; w < s.W; ++w) {
+      const std::complex<double> o = std::exp(std::complex<double>(lo[2 * w], lo[2 * w + 1]));
+      overlaps[2 * w] = o.real();
+      overlaps[2 * w + 1] = o.imag();
+    }
+  }
+  if (weights) download(weights, s.weights, s.W, c->stream);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_get_weights(afq_ctx* c, double* weights) { return afq_get_walkers(c, nullptr, nullptr, weights); }
+
+int afq_set_weights(afq_ctx* c, const double* weights) {
+  if (!c || !weights) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  upload(c->s.weights, weights, c->s.W, c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "sync");
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_propagate(afq_ctx* c, const double* draws, uint64_t seed, uint64_t step, double eshift,
+                  double* factors, int64_t* n_capped) {
+  if (!c) return AFQ_ERR_ARG;
+  if (!c->prepped || !c->have_walkers) return fail(c, AFQ_ERR_STATE, "afq_prep_prop and afq_set_walkers first");
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  afq::launch_zero_counter(s.ncap, c->stream);
+  if (draws) upload(s.draws, draws, (size_t)s.W * s.G, c->stream);
+  {
+    Stage t(c, afq::ST_FIELDS);
+    afq::launch_fields(s, draws ? s.draws : nullptr, seed, step, c->stream);
+    launch_check("fields");
+  }
+  {
+    Stage t(c, afq::ST_VHS);
+    afq::launch_vhs(s, c->stream);
+    launch_check("vhs");
+  }
+  {
+    Stage t(c, afq::ST_PROP);
+    afq::launch_propagate(s, c->stream);
+    launch_check("propagate");
+  }
+  {
+    Stage t(c, afq::ST_GREEN);
+    afq::launch_green(s, false, c->stream);
+    launch_check("green");
+  }
+  {
+    Stage t(c, afq::ST_WEIGHT);
+    afq::launch_weight(s, eshift, true, c->stream);
+    launch_check("weight");
+  }
+  {
+    Stage t(c, afq::ST_FB);
+    afq::launch_force_bias(s, c->stream);
+    launch_check("force_bias");
+  }
+  if (factors) download(factors, s.factors, s.W, c->stream);
+  if (n_capped) {
+    unsigned long long v = 0;
+    hip_check(hipMemcpyAsync(&v, s.ncap, sizeof(v), hipMemcpyDeviceToHost, c->stream), "D2H");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    *n_capped = (int64_t)v;
+  }
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_orthogonalize(afq_ctx* c) {
+  if (!c) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  Stage t(c, afq::ST_ORTHO);
+  afq::launch_qr(c->s, c->stream);
+  launch_check("qr");
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_measure(afq_ctx* c, int32_t mode, double delta, double* eloc, double* ham_ovlp) {
+  if (!c) return AFQ_ERR_ARG;
+  if (mode != AFQ_ENERGY_EXACT) return fail(c, AFQ_ERR_UNSUPPORTED, "energy mode not available");
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  {
+    Stage t(c, afq::ST_ENERGY);
+    if (s.GE > 0) afq::launch_exchange(s, nullptr, s.W, c->stream);
+    launch_check("exchange");
+    afq::launch_energy(s, nullptr, s.W, delta, c->stream);
+    launch_check("energy");
+  }
+  if (eloc) download(eloc, s.eloc, s.W, c->stream);
+  if (ham_ovlp) download(ham_ovlp, s.ho, 4 * (size_t)s.W, c->stream);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_ham_and_overlap(afq_ctx* c, const double* dets, int32_t ndet, int32_t nchol_ene, double* out) {
+  if (!c || !dets || !out || ndet <= 0) return c ? fail(c, AFQ_ERR_ARG, "bad arguments") : AFQ_ERR_ARG;
+  if (nchol_ene < 0 || nchol_ene > c->s.G) return fail(c, AFQ_ERR_ARG, "nchol_ene out of range");
+  AFQ_GUARD_BEGIN
+  // a temporary population of ndet walkers sharing the Hamiltonian buffers
+  DevState t = c->s;
+  t.W = ndet;
+  t.GE = nchol_ene;
+  alloc_walkers(t);
+  try {
+    upload(t.psi, dets, 2 * (size_t)ndet * t.ms, c->stream);
+    afq::launch_fill(t.weights, ndet, 1.0, c->stream);
+    afq::launch_green(t, true, c->stream);
+    afq::launch_logO_from_logdet(t, c->stream);
+    afq::launch_force_bias(t, c->stream);
+    if (t.GE > 0) afq::launch_exchange(t, nullptr, ndet, c->stream);
+    afq::launch_energy(t, nullptr, ndet, 0.0, c->stream);
+    launch_check("ham_and_overlap");
+    download(out, t.ho, 4 * (size_t)ndet, c->stream);
+  } catch (...) {
+    DevState z = t;
+    z.E = z.Lp = z.R = z.mf = nullptr;
+    z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
+    free_all(z);
+    throw;
+  }
+  DevState z = t;
+  z.E = z.Lp = z.R = z.mf = nullptr;
+  z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
+  free_all(z);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_force_bias(afq_ctx* c, double* fb) {
+  if (!c || !fb) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  download(fb, c->s.fb, 2 * (size_t)c->s.W * c->s.G, c->stream);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_overlap(afq_ctx* c, double* overlaps) { return afq_get_walkers(c, nullptr, overlaps, nullptr); }
+
+int afq_comb(afq_ctx* c, double zeta, double* total_weight) {
+  if (!c) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  {
+    Stage t(c, afq::ST_COMB);
+    afq::launch_comb(c->s, zeta, c->stream);
+    launch_check("comb");
+  }
+  if (total_weight) download(total_weight, c->s.cum + (c->s.W - 1), 1, c->stream);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_stage_times(afq_ctx* c, double* ms9, int32_t reset) {
+  if (!c) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  c->resolve();
+  for (int i = 0; i < afq::ST_COUNT; ++i) {
+    if (ms9) ms9[i] = c->stage_ms[i];
+    if (reset) c->stage_ms[i] = 0.0;
+  }
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_debug_gemm_f64(const double* A, const double* B, double* C, int32_t M, int32_t N, int32_t K) {
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0) return fail(nullptr, AFQ_ERR_ARG, "bad gemm arguments");
+  try {
+    double* dA = dalloc<double>((size_t)M * K);
+    double* dB = dalloc<double>((size_t)K * N);
+    double* dC = dalloc<double>((size_t)M * N);
+    upload(dA, A, (size_t)M * K, nullptr);
+    upload(dB, B, (size_t)K * N, nullptr);
+    afq::launch_debug_gemm(dA, dB, dC, M, N, K, nullptr);
+    launch_check("debug_gemm");
+    download(C, dC, (size_t)M * N, nullptr);
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);
+  } catch (const DevError& e) {
+    return fail(nullptr, e.code, e.what());
+  }
+  return AFQ_OK;
+}
+
+int afq_sync(afq_ctx* c) {
+  if (!c) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+}  // extern "C"

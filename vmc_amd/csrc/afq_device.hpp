@@ -1,0 +1,185 @@
+// Device building blocks for the CDNA4 (gfx950) phaseless-AFQMC kernels.
+//
+//  * MmaF64: a block-level FP64 MFMA tile engine.  One workgroup of
+//    64*WAVES_M*WAVES_N threads computes a BM x BN tile of C = A * B where the
+//    operands are produced element-wise by loader functors (so packed, strided,
+//    gathered or complex-embedded operands all become plain real GEMMs), staged
+//    through LDS in BK=16 slices with one slice of register prefetch, and
+//    accumulated with v_mfma_f64_16x16x4_f64.
+//  * small complex helpers and wavefront / block reductions (64-wide waves).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afq {
+
+typedef double __attribute__((ext_vector_type(4))) f64x4;
+typedef float __attribute__((ext_vector_type(4))) f32x4;
+
+struct cplx {
+  double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.im + b.re * r;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+__device__ __forceinline__ cplx clog(cplx a) { return {log(hypot(a.re, a.im)), atan2(a.im, a.re)}; }
+__device__ __forceinline__ cplx cexp(cplx a) {
+  const double m = exp(a.re);
+  double s, c;
+  sincos(a.im, &s, &c);
+  return {m * c, m * s};
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over the whole block (blockDim.x multiple of 64, <= 1024).  All threads
+// receive the result.  scratch: >= 16 doubles of LDS.
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < nw; ++i) t += scratch[i];
+  __syncthreads();
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA tile engine.
+//
+// v_mfma_f64_16x16x4_f64 operand layout (one 64-lane wave):
+//   A (16x4):  lane l supplies A[l % 16][l / 16]
+//   B (4x16):  lane l supplies B[l / 16][l % 16]
+//   D (16x16): lane l holds D[l/16 + 4*r][l % 16] in element r = 0..3
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WAVES_M, int WAVES_N, bool A_ROWFAST = false, bool B_COLFAST = true>
+struct MmaF64 {
+  static constexpr int BK = 16;
+  static constexpr int NT = 64 * WAVES_M * WAVES_N;
+  static constexpr int WM = BM / WAVES_M;
+  static constexpr int WN = BN / WAVES_N;
+  static constexpr int TM = WM / 16;
+  static constexpr int TN = WN / 16;
+  static constexpr int LDA = BM + 2;
+  static constexpr int LDB = BN + 2;
+  static constexpr int LDS_DOUBLES = BK * (LDA + LDB);
+  static constexpr int A_PER_T = (BM * BK) / NT;
+  static constexpr int B_PER_T = (BK * BN) / NT;
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+  static_assert((BM * BK) % NT == 0 && (BK * BN) % NT == 0, "tile not divisible by threads");
+
+  f64x4 acc[TM][TN];
+  int lane, wm, wn;
+
+  __device__ __forceinline__ MmaF64() {
+    lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    wm = wave / WAVES_N;
+    wn = wave % WAVES_N;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  }
+
+  __device__ __forceinline__ static void a_coord(int e, int& r, int& kk) {
+    if (A_ROWFAST) { r = e % BM; kk = e / BM; }
+    else { r = e / BK; kk = e % BK; }
+  }
+  __device__ __forceinline__ static void b_coord(int e, int& kk, int& c) {
+    if (B_COLFAST) { kk = e / BN; c = e % BN; }
+    else { kk = e % BK; c = e / BK; }
+  }
+
+  // la(r, k): A element (tile-relative row r in [0,BM), absolute k); must return 0 out of range.
+  // lb(k, c): B element (absolute k, tile-relative col c in [0,BN)).
+  template <class LA, class LB>
+  __device__ __forceinline__ void run(const LA& la, const LB& lb, int K, double* lds) {
+    double* As = lds;
+    double* Bs = lds + BK * LDA;
+    const int tid = threadIdx.x;
+    double ra[A_PER_T], rb[B_PER_T];
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < A_PER_T; ++q) {
+        int r, kk;
+        a_coord(tid + q * NT, r, kk);
+        ra[q] = la(r, k0 + kk);
+      }
+#pragma unroll
+      for (int q = 0; q < B_PER_T; ++q) {
+        int kk, c;
+        b_coord(tid + q * NT, kk, c);
+        rb[q] = lb(k0 + kk, c);
+      }
+    };
+    const int nst = (K + BK - 1) / BK;
+    if (nst > 0) fetch(0);
+    for (int s = 0; s < nst; ++s) {
+#pragma unroll
+      for (int q = 0; q < A_PER_T; ++q) {
+        int r, kk;
+        a_coord(tid + q * NT, r, kk);
+        As[kk * LDA + r] = ra[q];
+      }
+#pragma unroll
+      for (int q = 0; q < B_PER_T; ++q) {
+        int kk, c;
+        b_coord(tid + q * NT, kk, c);
+        Bs[kk * LDB + c] = rb[q];
+      }
+      __syncthreads();
+      if (s + 1 < nst) fetch((s + 1) * BK);
+#pragma unroll
+      for (int k4 = 0; k4 < BK / 4; ++k4) {
+        const int kr = k4 * 4 + (lane >> 4);
+        double a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) a[i] = As[kr * LDA + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) b[j] = Bs[kr * LDB + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+
+  // tile-relative coordinates of accumulator element acc[i][j][r]
+  __device__ __forceinline__ int row(int i, int r) const { return wm * WM + i * 16 + (lane >> 4) + 4 * r; }
+  __device__ __forceinline__ int col(int j) const { return wn * WN + j * 16 + (lane & 15); }
+
+  template <class EP>
+  __device__ __forceinline__ void store(const EP& ep) const {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ep(row(i, r), col(j), acc[i][j][r]);
+  }
+};
+
+}  // namespace afq

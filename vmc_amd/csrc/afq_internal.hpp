@@ -1,0 +1,102 @@
+// Internal declarations shared by the kernels (afq_kernels.hip) and the host
+// context / C ABI (afq_ctx.cpp).  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/vmc_afqmc.h"
+
+namespace afq {
+
+// One spin slot of the trial evaluation.  An RHF walker against an RHF trial
+// collapses to a single slot of multiplicity 2 (DQMC/RHF.cpp:80-107,150-177);
+// every other pairing uses two slots of multiplicity 1.
+struct Slot {
+  int n;         // electrons in the slot
+  int det;       // walker determinant it reads (0 or 1)
+  int phi;       // trial orbital set (0 up, 1 down)
+  int mult;      // multiplicity
+  int64_t koff;  // offset of this slot in a walker's theta / the rotated Cholesky rows (complex / real elems)
+};
+
+// Per-kernel device time buckets (afq_stage_times order).
+enum Stage { ST_FB = 0, ST_FIELDS, ST_VHS, ST_PROP, ST_GREEN, ST_WEIGHT, ST_ORTHO, ST_ENERGY, ST_COMB, ST_COUNT };
+
+struct DevState {
+  // sizes
+  int N = 0, na = 0, nb = 0, G = 0, GE = 0, W = 0;
+  int64_t T = 0;        // N(N+1)/2
+  int64_t ms = 0;       // complex elements per walker determinant set, N(na+nb)
+  int64_t Ktot = 0;     // complex elements of theta per walker, sum_s N n_s
+  int nslots = 0;
+  Slot slot[2];
+  int ndets_prop = 0;   // determinants propagated per walker (1 for an RHF walker)
+  int det_n[2] = {0, 0};
+  int64_t det_off[2] = {0, 0};  // complex offset of det d in a walker's serial block
+  bool walker_rhf = false, trial_rhf = false, ref_compat = false;
+  double ecore = 0.0, dt = 0.0, weight_cap = 100.0;
+  double prop_const_re = 0.0, prop_const_im = 0.0;
+
+  // Hamiltonian / trial (device)
+  double* E = nullptr;      // exp(-dt/2 h1'), N x N
+  double* Lp = nullptr;     // packed lower-triangle L, G x T (FP64, or float-rounded in ref_compat)
+  double* R = nullptr;      // rotated Cholesky, G x Ktot: [g][koff_s + k*n_s + i] = (Phi_s^T L_g)(i,k)
+  double* phi[2] = {nullptr, nullptr};   // N x n_s row-major
+  double* hphi[2] = {nullptr, nullptr};  // h1 Phi_s, N x n_s
+  double* mf = nullptr;     // G complex mean-field shifts
+
+  // walkers (device)
+  double* psi = nullptr;    // W x ms complex (serial layout)
+  double* theta = nullptr;  // W x Ktot complex
+  double* fb = nullptr;     // W x G complex
+  double* logO = nullptr;   // W complex: log of the trial overlap
+  double* e1 = nullptr;     // W x nslots complex one-body partial
+  double* logdet = nullptr; // W x nslots complex, from the last Green's-function evaluation
+  double* weights = nullptr;
+  double* factors = nullptr;
+  // comb double buffers
+  double* psi2 = nullptr; double* theta2 = nullptr; double* fb2 = nullptr;
+  double* logO2 = nullptr; double* e12 = nullptr;
+  // step scratch
+  double* fields = nullptr;  // W x G complex
+  double* sft = nullptr;     // W x 4: shift (complex), fbTerm (complex)
+  double* P = nullptr;       // W x 2 x T (VHS: sum_g Re/Im(field) L_g)
+  double* t0 = nullptr; double* t1 = nullptr; double* accb = nullptr;  // W x ms complex
+  double* draws = nullptr;   // W x G (replay)
+  double* Xp = nullptr;      // nslots x W x ngt complex exchange partials
+  int ngt_cap = 0;
+  double* eloc = nullptr;    // W
+  double* ho = nullptr;      // W x 4 {E*O, O}
+  double* cum = nullptr;     // W (comb)
+  int* idx = nullptr;        // W (comb)
+  unsigned long long* ncap = nullptr;
+};
+
+// kernel launchers (afq_kernels.hip)
+void launch_force_bias(const DevState& s, hipStream_t st);
+void launch_fields(const DevState& s, const double* draws, uint64_t seed, uint64_t step, hipStream_t st);
+void launch_vhs(const DevState& s, hipStream_t st);
+void launch_propagate(const DevState& s, hipStream_t st);
+void launch_green(const DevState& s, bool all_walkers, hipStream_t st);
+void launch_weight(const DevState& s, double eshift, bool update_weights, hipStream_t st);
+void launch_logO_from_logdet(const DevState& s, hipStream_t st);
+void launch_qr(const DevState& s, hipStream_t st);
+void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st);
+void launch_energy(const DevState& s, const int* sel, int nsel, double delta, hipStream_t st);
+void launch_comb(DevState& s, double zeta, hipStream_t st);
+void launch_copy_det0_to_det1(const DevState& s, double* dst, const double* src, hipStream_t st);
+
+void launch_pack_chol(const double* L, double* Lp, int N, int G, int64_t T, bool round_f32, hipStream_t st);
+void launch_rotate_chol(const double* L, const double* phi, double* R, int N, int n, int G, int64_t Ktot,
+                        int64_t koff, hipStream_t st);
+void launch_debug_gemm(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t st);
+void launch_fill(double* p, int64_t n, double v, hipStream_t st);
+void launch_zero_counter(unsigned long long* p, hipStream_t st);
+
+int exchange_tile_g(int n);   // g-blocks per exchange tile for slot size n (0: unsupported)
+int exchange_tile_w(int n);   // walkers per exchange tile
+
+}  // namespace afq

@@ -1,0 +1,920 @@
+// HIP kernels of the phaseless-AFQMC walker step for gfx950 (FP64 path).
+//
+// Every dense contraction is a real GEMM on the FP64 MFMA tile engine
+// (afq_device.hpp) with loaders that read the operand in its storage layout:
+//   force bias    C[(w,c), g]  = sum_k theta[w][k][c] R[g][k]          (DQMC/RHF.cpp:54-61, DQMC/UHF.cpp:45-53)
+//   VHS           P[(w,c), t]  = sum_g field[w][g][c] Lp[g][t]          (DQMC/DQMCWalker.cpp:406-421)
+//   one-body      E psi, batched over walkers folded into columns        (DQMC/DQMCWalker.cpp:482-491)
+//   Taylor term   V t with complex V embedded as [A_re | A_im] x [[B_re,B_im],[-B_im,B_re]]
+//                                                                        (DQMC/DQMCWalker.cpp:424-431)
+//   Green         S = Phi^T psi, S^-1 by Gauss-Jordan in LDS, theta = psi S^-1
+//                                                                        (DQMC/RHF.cpp:27-61,104-131)
+//   exchange      f = R_g theta per (g, walker) block, sum_ij f_ij f_ji reduced from
+//                 the LDS-staged output tile without writing f           (DQMC/UHF.cpp:94-100)
+#include <float.h>
+
+#include <algorithm>
+#include <utility>
+
+#include "afq_device.hpp"
+#include "afq_internal.hpp"
+
+namespace afq {
+
+static inline unsigned cdivu(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ force bias
+struct FbA {  // A((w,c), k) = theta[w][k][c]
+  const double* th; int64_t Ktot; int row0, nrow;
+  __device__ double operator()(int r, int k) const {
+    const int q = row0 + r;
+    if (q >= nrow || k >= Ktot) return 0.0;
+    return th[(int64_t)(q >> 1) * 2 * Ktot + 2 * (int64_t)k + (q & 1)];
+  }
+};
+struct FbB {  // B(k, g) = R[g][k]
+  const double* R; int64_t Ktot; int g0, G;
+  __device__ double operator()(int k, int c) const {
+    const int g = g0 + c;
+    if (g >= G || k >= Ktot) return 0.0;
+    return R[(int64_t)g * Ktot + k];
+  }
+};
+
+constexpr int FB_BM = 128, FB_BN = 128;
+__global__ __launch_bounds__(256) void k_force_bias(const double* R, const double* th, double* fb,
+                                                    int G, int W, int64_t Ktot, double alpha) {
+  using Eng = MmaF64<FB_BM, FB_BN, 2, 2, false, false>;
+  __shared__ double lds[Eng::LDS_DOUBLES];
+  const int row0 = blockIdx.y * FB_BM, g0 = blockIdx.x * FB_BN;
+  Eng e;
+  e.run(FbA{th, Ktot, row0, 2 * W}, FbB{R, Ktot, g0, G}, (int)Ktot, lds);
+  e.store([&](int r, int c, double v) {
+    const int q = row0 + r, g = g0 + c;
+    if (q < 2 * W && g < G) fb[(int64_t)(q >> 1) * 2 * G + 2 * g + (q & 1)] = alpha * v;
+  });
+}
+
+void launch_force_bias(const DevState& s, hipStream_t st) {
+  dim3 grid(cdivu(s.G, FB_BN), cdivu(2 * (int64_t)s.W, FB_BM));
+  hipLaunchKernelGGL(k_force_bias, grid, dim3(256), 0, st, s.R, s.theta, s.fb, s.G, s.W, s.Ktot,
+                     (double)s.slot[0].mult);
+}
+
+// ------------------------------------------------------------------ fields
+// Philox4x32-10 (Salmon et al., SC'11), counter = (pair index, step lo, step hi, walker),
+// key = seed; two 53-bit uniforms per call, Box-Muller to two normals.
+__device__ __host__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int i = 0; i < 10; ++i) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[0] = n0; c[1] = (uint32_t)p1; c[2] = n2; c[3] = (uint32_t)p0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ inline double philox_normal(uint64_t seed, int w, uint64_t step, int g) {
+  uint32_t c[4] = {(uint32_t)(g >> 1), (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)w};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t a = ((uint64_t)c[0] << 32) | c[1], b = ((uint64_t)c[2] << 32) | c[3];
+  const double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+  const double u2 = (double)(b >> 11) * 0x1.0p-53;          // [0, 1)
+  const double r = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincospi(2.0 * u2, &sn, &cs);
+  return (g & 1) ? r * sn : r * cs;
+}
+
+// field loop of DQMC/DQMCWalker.cpp:451-457 for every live walker
+__global__ __launch_bounds__(256) void k_fields(const double* fb, const double* mf, const double* draws,
+                                                const double* weights, double* fields, double* sft,
+                                                int G, double dt, int ref_compat, uint64_t seed,
+                                                uint64_t step) {
+  __shared__ double red[16];
+  const int w = blockIdx.x;
+  if (weights[w] <= 1e-8) return;
+  const double sq = sqrt(dt);
+  double sh_re = 0, sh_im = 0, ft_re = 0, ft_im = 0;
+  for (int g = threadIdx.x; g < G; g += blockDim.x) {
+    const double x = draws ? draws[(int64_t)w * G + g] : philox_normal(seed, w, step, g);
+    const double fr = fb[(int64_t)w * 2 * G + 2 * g], fi = fb[(int64_t)w * 2 * G + 2 * g + 1];
+    const double mr = mf[2 * g], mi = mf[2 * g + 1];
+    // s = -sqrt(dt) (i fb - mf)
+    const double s_re = -sq * (-fi - mr), s_im = -sq * (fr - mi);
+    double p_re = x - s_re, p_im = -s_im;
+    // shift += (x - s) mf ; fbTerm += x s - s^2/2
+    sh_re += p_re * mr - p_im * mi;
+    sh_im += p_re * mi + p_im * mr;
+    ft_re += x * s_re - 0.5 * (s_re * s_re - s_im * s_im);
+    ft_im += x * s_im - s_re * s_im;
+    if (ref_compat) { p_re = (double)(float)p_re; p_im = (double)(float)p_im; }
+    fields[(int64_t)w * 2 * G + 2 * g] = p_re;
+    fields[(int64_t)w * 2 * G + 2 * g + 1] = p_im;
+  }
+  sh_re = block_sum(sh_re, red);
+  sh_im = block_sum(sh_im, red);
+  ft_re = block_sum(ft_re, red);
+  ft_im = block_sum(ft_im, red);
+  if (threadIdx.x == 0) {
+    sft[4 * w] = sh_re; sft[4 * w + 1] = sh_im; sft[4 * w + 2] = ft_re; sft[4 * w + 3] = ft_im;
+  }
+}
+
+void launch_fields(const DevState& s, const double* draws, uint64_t seed, uint64_t step, hipStream_t st) {
+  hipLaunchKernelGGL(k_fields, dim3(s.W), dim3(256), 0, st, s.fb, s.mf, draws, s.weights, s.fields,
+                     s.sft, s.G, s.dt, (int)s.ref_compat, seed, step);
+}
+
+// ------------------------------------------------------------------ VHS
+struct VhsA {  // A((w,c), g) = field[w][g][c]
+  const double* f; int G; int row0, nrow;
+  __device__ double operator()(int r, int k) const {
+    const int q = row0 + r;
+    if (q >= nrow || k >= G) return 0.0;
+    return f[(int64_t)(q >> 1) * 2 * G + 2 * k + (q & 1)];
+  }
+};
+struct VhsB {  // B(g, t) = Lp[g][t]
+  const double* Lp; int64_t T; int G; int64_t t0;
+  __device__ double operator()(int k, int c) const {
+    const int64_t t = t0 + c;
+    if (k >= G || t >= T) return 0.0;
+    return Lp[(int64_t)k * T + t];
+  }
+};
+constexpr int VHS_BM = 128, VHS_BN = 128;
+__global__ __launch_bounds__(256) void k_vhs(const double* fields, const double* Lp, double* P,
+                                             const double* weights, int G, int W, int64_t T,
+                                             int ref_compat) {
+  using Eng = MmaF64<VHS_BM, VHS_BN, 2, 2, false, true>;
+  __shared__ double lds[Eng::LDS_DOUBLES];
+  const int row0 = blockIdx.y * VHS_BM;
+  const int64_t t0 = (int64_t)blockIdx.x * VHS_BN;
+  // skip tiles whose walkers are all dead
+  bool any = false;
+  for (int q = row0; q < min(row0 + VHS_BM, 2 * W); q += 2) any |= weights[q >> 1] > 1e-8;
+  if (!any) return;
+  Eng e;
+  e.run(VhsA{fields, G, row0, 2 * W}, VhsB{Lp, T, G, t0}, G, lds);
+  e.store([&](int r, int c, double v) {
+    const int q = row0 + r;
+    const int64_t t = t0 + c;
+    if (q < 2 * W && t < T) P[(int64_t)q * T + t] = ref_compat ? (double)(float)v : v;
+  });
+}
+
+void launch_vhs(const DevState& s, hipStream_t st) {
+  dim3 grid(cdivu(s.T, VHS_BN), cdivu(2 * (int64_t)s.W, VHS_BM));
+  hipLaunchKernelGGL(k_vhs, grid, dim3(256), 0, st, s.fields, s.Lp, s.P, s.weights, s.G, s.W, s.T,
+                     (int)s.ref_compat);
+}
+
+// ------------------------------------------------------------------ propagation
+// out = E * src for determinant d of every walker (walkers folded into columns).
+struct ExpA {
+  const double* E; int N; int row0;
+  __device__ double operator()(int r, int k) const {
+    const int i = row0 + r;
+    return (i < N && k < N) ? E[(int64_t)i * N + k] : 0.0;
+  }
+};
+struct DetColsB {  // B(k, q) = src[w][det d][k][cc], q = w*2n + cc
+  const double* src; int64_t wstride, doff; int N, n2, q0, ncol;
+  __device__ double operator()(int k, int c) const {
+    const int q = q0 + c;
+    if (q >= ncol || k >= N) return 0.0;
+    const int w = q / n2, cc = q - w * n2;
+    return src[(int64_t)w * wstride + doff + (int64_t)k * n2 + cc];
+  }
+};
+constexpr int EX_BM = 64, EX_BN = 128;
+// mode 0: dst1 = dst2 = E src (start of the Taylor series); mode 1: dst1 = E src for live walkers
+__global__ __launch_bounds__(256) void k_expmul(const double* E, const double* src, double* dst1,
+                                                double* dst2, const double* weights, int N, int n,
+                                                int W, int64_t wstride, int64_t doff, int mode) {
+  using Eng = MmaF64<EX_BM, EX_BN, 2, 2, false, true>;
+  __shared__ double lds[Eng::LDS_DOUBLES];
+  const int row0 = blockIdx.y * EX_BM, q0 = blockIdx.x * EX_BN;
+  const int n2 = 2 * n, ncol = W * n2;
+  Eng e;
+  e.run(ExpA{E, N, row0}, DetColsB{src, wstride, doff, N, n2, q0, ncol}, N, lds);
+  e.store([&](int r, int c, double v) {
+    const int i = row0 + r, q = q0 + c;
+    if (i >= N || q >= ncol) return;
+    const int w = q / n2, cc = q - w * n2;
+    if (weights[w] <= 1e-8) return;
+    const int64_t a = (int64_t)w * wstride + doff + (int64_t)i * n2 + cc;
+    dst1[a] = v;
+    if (mode == 0) dst2[a] = v;
+  });
+}
+
+// t_dst = (V t_src)/p, acc += t_dst for one walker per blockIdx.z; V = i sqrt(dt) (P_re + i P_im)
+struct VA {  // [A_re | A_im], A_re = -sq P_im, A_im = sq P_re, symmetric packed
+  const double* Pre; const double* Pim; int N; int row0; double sq;
+  __device__ double operator()(int r, int k) const {
+    const int i = row0 + r;
+    if (i >= N || k >= 2 * N) return 0.0;
+    const int kk = k < N ? k : k - N;
+    const int a = i > kk ? i : kk, b = i > kk ? kk : i;
+    const int64_t idx = (int64_t)a * (a + 1) / 2 + b;
+    return k < N ? -sq * Pim[idx] : sq * Pre[idx];
+  }
+};
+struct TB {  // [[B_re, B_im], [-B_im, B_re]] over the walker's propagated dets, interleaved cols
+  const double* src; int N; int n2a, n2b; int64_t offb; int q0, ncol;
+  __device__ double operator()(int k, int c) const {
+    const int q = q0 + c;
+    if (q >= ncol || k >= 2 * N) return 0.0;
+    const double* base;
+    int n2, cl;
+    if (q < n2a) { base = src; n2 = n2a; cl = q; }
+    else { base = src + offb; n2 = n2b; cl = q - n2a; }
+    if (k < N) return base[(int64_t)k * n2 + cl];
+    const int comp = cl & 1, j2 = cl - comp;
+    const double* row = base + (int64_t)(k - N) * n2 + j2;
+    return comp ? row[0] : -row[1];
+  }
+};
+constexpr int VM_BM = 64, VM_BN = 128;
+__global__ __launch_bounds__(256) void k_vmul(const double* P, const double* tsrc, double* tdst,
+                                              double* acc, const double* weights, int N, int64_t T,
+                                              int n2a, int n2b, int64_t offb, int64_t wstride,
+                                              double sq, double inv_p) {
+  using Eng = MmaF64<VM_BM, VM_BN, 2, 2, false, true>;
+  __shared__ double lds[Eng::LDS_DOUBLES];
+  const int w = blockIdx.z;
+  if (weights[w] <= 1e-8) return;
+  const int row0 = blockIdx.y * VM_BM, q0 = blockIdx.x * VM_BN;
+  const int ncol = n2a + n2b;
+  const double* Pre = P + (int64_t)(2 * w) * T;
+  const double* Pim = Pre + T;
+  const double* src = tsrc + (int64_t)w * wstride;
+  Eng e;
+  e.run(VA{Pre, Pim, N, row0, sq}, TB{src, N, n2a, n2b, offb, q0, ncol}, 2 * N, lds);
+  double* dst = tdst + (int64_t)w * wstride;
+  double* ac = acc + (int64_t)w * wstride;
+  e.store([&](int r, int c, double v) {
+    const int i = row0 + r, q = q0 + c;
+    if (i >= N || q >= ncol) return;
+    const int64_t a = q < n2a ? (int64_t)i * n2a + q : offb + (int64_t)i * n2b + (q - n2a);
+    const double x = v * inv_p;
+    dst[a] = x;
+    ac[a] += x;
+  });
+}
+
+// psi <- E (sum_{k<=5} V^k / k!) E psi  for every live walker
+void launch_propagate(const DevState& s, hipStream_t st) {
+  const int64_t ws = 2 * s.ms;
+  for (int d = 0; d < s.ndets_prop; ++d) {
+    const int n = s.det_n[d];
+    dim3 grid(cdivu((int64_t)s.W * 2 * n, EX_BN), cdivu(s.N, EX_BM));
+    hipLaunchKernelGGL(k_expmul, grid, dim3(256), 0, st, s.E, s.psi, s.t0, s.accb, s.weights, s.N, n,
+                       s.W, ws, 2 * s.det_off[d], 0);
+  }
+  const int n2a = 2 * s.det_n[0];
+  const int n2b = s.ndets_prop > 1 ? 2 * s.det_n[1] : 0;
+  const int64_t offb = 2 * s.det_off[1];
+  const double sq = sqrt(s.dt);
+  double* src = s.t0;
+  double* dst = s.t1;
+  for (int p = 1; p <= 5; ++p) {
+    dim3 grid(cdivu(n2a + n2b, VM_BN), cdivu(s.N, VM_BM), s.W);
+    hipLaunchKernelGGL(k_vmul, grid, dim3(256), 0, st, s.P, src, dst, s.accb, s.weights, s.N, s.T, n2a,
+                       n2b, offb, ws, sq, 1.0 / p);
+    double* t = src; src = dst; dst = t;
+  }
+  for (int d = 0; d < s.ndets_prop; ++d) {
+    const int n = s.det_n[d];
+    dim3 grid(cdivu((int64_t)s.W * 2 * n, EX_BN), cdivu(s.N, EX_BM));
+    hipLaunchKernelGGL(k_expmul, grid, dim3(256), 0, st, s.E, s.accb, s.psi, nullptr, s.weights, s.N,
+                       n, s.W, ws, 2 * s.det_off[d], 1);
+  }
+}
+
+// ------------------------------------------------------------------ Green's function
+constexpr int GN_MAX = 64;
+struct GreenArgs {
+  const double* psi; double* theta; double* logdet; double* e1; const double* weights;
+  const double* phi0; const double* phi1; const double* hphi0; const double* hphi1;
+  int64_t ms, Ktot; int64_t det_off0, det_off1; int N, nslots; Slot s0, s1; int all;
+};
+
+__global__ __launch_bounds__(256) void k_green(GreenArgs a) {
+  using ES = MmaF64<64, 128, 2, 2, true, true>;
+  using ET = MmaF64<64, 128, 2, 2, false, true>;
+  constexpr int STAGE = ES::LDS_DOUBLES > ET::LDS_DOUBLES ? ES::LDS_DOUBLES : ET::LDS_DOUBLES;
+  __shared__ double stage[STAGE];
+  __shared__ cplx Sm[GN_MAX * GN_MAX];
+  __shared__ cplx fcol[GN_MAX];
+  __shared__ double red[16];
+  __shared__ int piv[GN_MAX];
+  __shared__ int pidx;
+  const int w = blockIdx.x, si = blockIdx.y;
+  if (!a.all && a.weights[w] <= 1e-8) return;
+  const Slot sl = si == 0 ? a.s0 : a.s1;
+  const int n = sl.n, N = a.N, n2 = 2 * n, tid = threadIdx.x;
+  const double* psi = a.psi + (int64_t)w * 2 * a.ms + 2 * (sl.det == 0 ? a.det_off0 : a.det_off1);
+  const double* phi = sl.phi == 0 ? a.phi0 : a.phi1;
+  const double* hphi = sl.phi == 0 ? a.hphi0 : a.hphi1;
+  double* Sd = reinterpret_cast<double*>(Sm);
+  // S = Phi^T psi  (n x n complex, row stride 2*GN_MAX doubles)
+  {
+    ES e;
+    e.run([&](int r, int k) { return (r < n && k < N) ? phi[(int64_t)k * n + r] : 0.0; },
+          [&](int k, int c) { return (k < N && c < n2) ? psi[(int64_t)k * n2 + c] : 0.0; }, N, stage);
+    e.store([&](int r, int c, double v) {
+      if (r < n && c < n2) Sd[r * 2 * GN_MAX + c] = v;
+    });
+  }
+  __syncthreads();
+  // in-place Gauss-Jordan inversion with partial pivoting; log det from the pivots
+  double ld_re = 0.0, ld_im = 0.0;
+  int nswap = 0;
+  for (int k = 0; k < n; ++k) {
+    if (tid < 64) {
+      int i = k + tid;
+      double v = -1.0;
+      if (i < n) { const cplx x = Sm[i * GN_MAX + k]; v = x.re * x.re + x.im * x.im; }
+      else i = n;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(i, off, 64);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+      }
+      if (tid == 0) { pidx = i; piv[k] = i; }
+    }
+    __syncthreads();
+    const int p = pidx;
+    if (p != k)
+      for (int j = tid; j < n; j += blockDim.x) {
+        const cplx t = Sm[k * GN_MAX + j];
+        Sm[k * GN_MAX + j] = Sm[p * GN_MAX + j];
+        Sm[p * GN_MAX + j] = t;
+      }
+    __syncthreads();
+    const cplx pv = Sm[k * GN_MAX + k];
+    if (tid == 0) {
+      const cplx l = clog(pv);
+      ld_re += l.re; ld_im += l.im;
+      if (p != k) ++nswap;
+    }
+    const cplx pinv = cdiv(cplx{1.0, 0.0}, pv);
+    for (int i = tid; i < n; i += blockDim.x) fcol[i] = Sm[i * GN_MAX + k];
+    __syncthreads();
+    for (int j = tid; j < n; j += blockDim.x)
+      Sm[k * GN_MAX + j] = (j == k) ? pinv : cmul(Sm[k * GN_MAX + j], pinv);
+    __syncthreads();
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      const int i = e / n, j = e - i * n;
+      if (i == k) continue;
+      const cplx base = (j == k) ? cplx{0.0, 0.0} : Sm[i * GN_MAX + j];
+      Sm[i * GN_MAX + j] = csub(base, cmul(fcol[i], Sm[k * GN_MAX + j]));
+    }
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int i = tid; i < n; i += blockDim.x) {
+        const cplx t = Sm[i * GN_MAX + k];
+        Sm[i * GN_MAX + k] = Sm[i * GN_MAX + p];
+        Sm[i * GN_MAX + p] = t;
+      }
+    __syncthreads();
+  }
+  // theta = psi S^-1 (N x n complex), one-body e1 = sum theta o (h1 Phi)
+  double e1re = 0.0, e1im = 0.0;
+  double* th = a.theta + (int64_t)w * 2 * a.Ktot + 2 * sl.koff;
+  for (int r0 = 0; r0 < N; r0 += 64) {
+    ET e;
+    e.run(
+        [&](int r, int k) {
+          const int rr = r0 + r;
+          if (rr >= N || k >= n2) return 0.0;
+          return k < n ? psi[(int64_t)rr * n2 + 2 * k] : psi[(int64_t)rr * n2 + 2 * (k - n) + 1];
+        },
+        [&](int k, int c) {
+          if (k >= n2 || c >= n2) return 0.0;
+          const int j = c >> 1, comp = c & 1;
+          if (k < n) { const cplx v = Sm[k * GN_MAX + j]; return comp ? v.im : v.re; }
+          const cplx v = Sm[(k - n) * GN_MAX + j];
+          return comp ? v.re : -v.im;
+        },
+        n2, stage);
+    e.store([&](int r, int c, double v) {
+      const int rr = r0 + r;
+      if (rr < N && c < n2) {
+        th[(int64_t)rr * n2 + c] = v;
+        const double h = hphi[(int64_t)rr * n + (c >> 1)];
+        if (c & 1) e1im += v * h; else e1re += v * h;
+      }
+    });
+  }
+  e1re = block_sum(e1re, red);
+  e1im = block_sum(e1im, red);
+  if (tid == 0) {
+    if (nswap & 1) ld_im += M_PI;
+    a.logdet[2 * ((int64_t)w * a.nslots + si)] = ld_re;
+    a.logdet[2 * ((int64_t)w * a.nslots + si) + 1] = ld_im;
+    a.e1[2 * ((int64_t)w * a.nslots + si)] = e1re;
+    a.e1[2 * ((int64_t)w * a.nslots + si) + 1] = e1im;
+  }
+}
+
+void launch_green(const DevState& s, bool all, hipStream_t st) {
+  GreenArgs a;
+  a.psi = s.psi; a.theta = s.theta; a.logdet = s.logdet; a.e1 = s.e1; a.weights = s.weights;
+  a.phi0 = s.phi[0]; a.phi1 = s.phi[1]; a.hphi0 = s.hphi[0]; a.hphi1 = s.hphi[1];
+  a.ms = s.ms; a.Ktot = s.Ktot; a.det_off0 = s.det_off[0]; a.det_off1 = s.det_off[1];
+  a.N = s.N; a.nslots = s.nslots; a.s0 = s.slot[0]; a.s1 = s.slot[s.nslots > 1 ? 1 : 0]; a.all = all;
+  hipLaunchKernelGGL(k_green, dim3(s.W, s.nslots), dim3(256), 0, st, a);
+}
+
+// ------------------------------------------------------------------ weights
+// phaseless factor of DQMC/DQMCWalker.cpp:495-502 and the driver's update/cap
+// (DQMC/MixedEstimator.cpp:461-468), with log overlaps.
+__global__ void k_weight(double* weights, double* factors, double* logO, const double* logdet,
+                         const double* sft, int W, int nslots, int m0, int m1, double dt, double eshift,
+                         double pc_re, double cap, unsigned long long* ncap, int update) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  double f = 1.0;
+  if (weights[w] > 1e-8) {
+    double lo_re = m0 * logdet[2 * (int64_t)w * nslots], lo_im = m0 * logdet[2 * (int64_t)w * nslots + 1];
+    if (nslots > 1) {
+      lo_re += m1 * logdet[2 * ((int64_t)w * nslots + 1)];
+      lo_im += m1 * logdet[2 * ((int64_t)w * nslots + 1) + 1];
+    }
+    const double dre = lo_re - logO[2 * w], dim = lo_im - logO[2 * w + 1];
+    const double sq = sqrt(dt);
+    const double zre = -sq * sft[4 * w] + sft[4 * w + 2] + dt * (eshift + pc_re) + dre;
+    const double zth = -sq * sft[4 * w + 1] + dim;
+    f = exp(zre) * cos(zth);
+    if (!(f >= 1.e-3) || f > 100.) f = 0.0;
+    if (update) weights[w] *= f;
+    logO[2 * w] = lo_re;
+    logO[2 * w + 1] = lo_im;
+  }
+  if (update && weights[w] > cap) {
+    weights[w] = 0.0;
+    atomicAdd(ncap, 1ull);
+  }
+  factors[w] = f;
+}
+
+void launch_weight(const DevState& s, double eshift, bool update, hipStream_t st) {
+  hipLaunchKernelGGL(k_weight, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.weights, s.factors, s.logO,
+                     s.logdet, s.sft, s.W, s.nslots, s.slot[0].mult, s.nslots > 1 ? s.slot[1].mult : 0,
+                     s.dt, eshift, s.prop_const_re, s.weight_cap, s.ncap, (int)update);
+}
+
+__global__ void k_logO(double* logO, const double* logdet, int W, int nslots, int m0, int m1) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  double re = m0 * logdet[2 * (int64_t)w * nslots], im = m0 * logdet[2 * (int64_t)w * nslots + 1];
+  if (nslots > 1) {
+    re += m1 * logdet[2 * ((int64_t)w * nslots + 1)];
+    im += m1 * logdet[2 * ((int64_t)w * nslots + 1) + 1];
+  }
+  logO[2 * w] = re;
+  logO[2 * w + 1] = im;
+}
+
+void launch_logO_from_logdet(const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_logO, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.logO, s.logdet, s.W, s.nslots,
+                     s.slot[0].mult, s.nslots > 1 ? s.slot[1].mult : 0);
+}
+
+// ------------------------------------------------------------------ Householder QR
+// Thin Householder QR with Eigen's conventions (eigen/Eigen/src/Householder/Householder.h:63-92,
+// eigen/Eigen/src/QR/HouseholderQR.h:267-297); psi_d := Q, logO -= log prod diag R
+// (DQMC/DQMCWalker.cpp:295-326).  One workgroup per walker, both determinants.
+__global__ __launch_bounds__(256) void k_qr(double* psi, double* Qs, double* logO, int N, int ndets,
+                                            int na, int nb, int64_t offb, int64_t ms, int rhf_walker) {
+  __shared__ cplx tau[GN_MAX];
+  __shared__ cplx tmp[GN_MAX];
+  __shared__ double red[16];
+  __shared__ double sbeta;
+  __shared__ cplx sden;
+  __shared__ int szero;
+  const int w = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwv = blockDim.x >> 6;
+  double lf_re = 0.0, lf_im = 0.0;
+  for (int d = 0; d < ndets; ++d) {
+    const int n = d == 0 ? na : nb;
+    cplx* A = reinterpret_cast<cplx*>(psi + 2 * ((int64_t)w * ms + (d == 0 ? 0 : offb)));
+    cplx* Q = reinterpret_cast<cplx*>(Qs + 2 * ((int64_t)w * ms + (d == 0 ? 0 : offb)));
+    for (int k = 0; k < n; ++k) {
+      double t = 0.0;
+      for (int i = k + 1 + tid; i < N; i += blockDim.x) {
+        const cplx x = A[(int64_t)i * n + k];
+        t += x.re * x.re + x.im * x.im;
+      }
+      const double tail = block_sum(t, red);
+      if (tid == 0) {
+        const cplx c0 = A[(int64_t)k * n + k];
+        if (tail <= DBL_MIN && c0.im * c0.im <= DBL_MIN) {
+          tau[k] = cplx{0.0, 0.0};
+          sbeta = c0.re;
+          szero = 1;
+        } else {
+          double beta = sqrt(c0.re * c0.re + c0.im * c0.im + tail);
+          if (c0.re >= 0.0) beta = -beta;
+          sbeta = beta;
+          sden = cplx{c0.re - beta, c0.im};
+          const cplx tt = cdiv(cplx{beta - c0.re, -c0.im}, cplx{beta, 0.0});
+          tau[k] = cplx{tt.re, -tt.im};
+          szero = 0;
+        }
+      }
+      __syncthreads();
+      const double beta = sbeta;
+      for (int i = k + 1 + tid; i < N; i += blockDim.x)
+        A[(int64_t)i * n + k] = szero ? cplx{0.0, 0.0} : cdiv(A[(int64_t)i * n + k], sden);
+      if (tid == 0) {
+        A[(int64_t)k * n + k] = cplx{beta, 0.0};
+        lf_re += log(fabs(beta));
+        if (beta < 0.0) lf_im += M_PI;
+      }
+      __syncthreads();
+      // tmp_j = A(k,j) + sum_{i>k} conj(v_i) A(i,j), j > k
+      for (int j = k + 1 + wave; j < n; j += nwv) {
+        double sr = 0.0, si = 0.0;
+        for (int i = k + 1 + lane; i < N; i += 64) {
+          const cplx v = A[(int64_t)i * n + k], x = A[(int64_t)i * n + j];
+          sr += v.re * x.re + v.im * x.im;
+          si += v.re * x.im - v.im * x.re;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) {
+          const cplx akj = A[(int64_t)k * n + j];
+          tmp[j] = cplx{akj.re + sr, akj.im + si};
+        }
+      }
+      __syncthreads();
+      const cplx tk = tau[k];
+      const int nc = n - k - 1;
+      for (int e = tid; e < (N - k) * nc; e += blockDim.x) {
+        const int i = k + e / nc, j = k + 1 + e % nc;
+        const cplx v = i == k ? cplx{1.0, 0.0} : A[(int64_t)i * n + k];
+        const cplx u = cmul(tk, cmul(v, tmp[j]));
+        A[(int64_t)i * n + j] = csub(A[(int64_t)i * n + j], u);
+      }
+      __syncthreads();
+    }
+    // Q = H_0^* ... H_{n-1}^* [I;0],  H_k^* = I - conj(tau_k) v v^*
+    for (int e = tid; e < N * n; e += blockDim.x) {
+      const int i = e / n, j = e - i * n;
+      Q[e] = cplx{i == j ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    for (int k = n - 1; k >= 0; --k) {
+      const cplx tk = cplx{tau[k].re, -tau[k].im};
+      for (int j = k + wave; j < n; j += nwv) {
+        double sr = 0.0, si = 0.0;
+        for (int i = k + 1 + lane; i < N; i += 64) {
+          const cplx v = A[(int64_t)i * n + k], x = Q[(int64_t)i * n + j];
+          sr += v.re * x.re + v.im * x.im;
+          si += v.re * x.im - v.im * x.re;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) {
+          const cplx qkj = Q[(int64_t)k * n + j];
+          tmp[j] = cplx{qkj.re + sr, qkj.im + si};
+        }
+      }
+      __syncthreads();
+      const int nc = n - k;
+      for (int e = tid; e < (N - k) * nc; e += blockDim.x) {
+        const int i = k + e / nc, j = k + e % nc;
+        const cplx v = i == k ? cplx{1.0, 0.0} : A[(int64_t)i * n + k];
+        Q[(int64_t)i * n + j] = csub(Q[(int64_t)i * n + j], cmul(tk, cmul(v, tmp[j])));
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < N * n; e += blockDim.x) A[e] = Q[e];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double m = rhf_walker ? 2.0 : 1.0;
+    logO[2 * w] -= m * lf_re;
+    logO[2 * w + 1] -= m * lf_im;
+  }
+}
+
+void launch_qr(const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_qr, dim3(s.W), dim3(256), 0, st, s.psi, s.t0, s.logO, s.N, s.ndets_prop,
+                     s.det_n[0], s.det_n[1], s.det_off[1], s.ms, (int)s.walker_rhf);
+}
+
+// ------------------------------------------------------------------ exchange
+constexpr int XC_BM = 128, XC_BN = 128;
+int exchange_tile_g(int n) { return (n >= 1 && n <= XC_BN / 2) ? XC_BM / n : 0; }
+int exchange_tile_w(int n) { return (n >= 1 && n <= XC_BN / 2) ? XC_BN / (2 * n) : 0; }
+
+struct XcA {  // A((g,i), k) = R_{s,g}(i, k) = R[g][koff + k*n + i]
+  const double* R; int64_t Ktot, koff; int n, gt, g0, GE, N;
+  __device__ double operator()(int r, int k) const {
+    const int gl = r / n, i = r - gl * n, g = g0 + gl;
+    if (gl >= gt || g >= GE || k >= N) return 0.0;
+    return R[(int64_t)g * Ktot + koff + (int64_t)k * n + i];
+  }
+};
+struct XcB {  // B(k, (w,j,c)) = theta_s[w][k][j][c]
+  const double* th; const int* sel; int64_t Ktot, koff; int n, wt, w0, nw, N;
+  __device__ double operator()(int k, int c) const {
+    const int n2 = 2 * n, wl = c / n2, cc = c - wl * n2;
+    if (wl >= wt || w0 + wl >= nw || k >= N) return 0.0;
+    const int w = sel ? sel[w0 + wl] : w0 + wl;
+    return th[(int64_t)w * 2 * Ktot + 2 * koff + (int64_t)k * n2 + cc];
+  }
+};
+
+// Xp[s][w][gtile] = sum_{g in tile} sum_ij f_ij f_ji,  f = R_{s,g} theta_s
+__global__ __launch_bounds__(256) void k_exchange(const double* R, const double* theta, const int* sel,
+                                                  double* Xp, int N, int GE, int nw, int W, int64_t Ktot,
+                                                  Slot s0, Slot s1, int ngt) {
+  using Eng = MmaF64<XC_BM, XC_BN, 2, 2, true, true>;
+  constexpr int FT = XC_BM * XC_BN;
+  constexpr int LDSN = Eng::LDS_DOUBLES > FT ? Eng::LDS_DOUBLES : FT;
+  __shared__ double lds[LDSN];
+  const Slot sl = blockIdx.z == 0 ? s0 : s1;
+  const int n = sl.n, gt = XC_BM / n, wt = XC_BN / (2 * n);
+  const int g0 = blockIdx.x * gt, w0 = blockIdx.y * wt;
+  Eng e;
+  e.run(XcA{R, Ktot, sl.koff, n, gt, g0, GE, N}, XcB{theta, sel, Ktot, sl.koff, n, wt, w0, nw, N}, N, lds);
+  // stage the output tile (row-major XC_BM x XC_BN) in LDS
+  e.store([&](int r, int c, double v) { lds[r * XC_BN + c] = v; });
+  __syncthreads();
+  // per (block, row i): sum_j f_ij f_ji (complex)
+  const int nrows = gt * wt * n;
+  const int tid = threadIdx.x;
+  constexpr int MAXR = (XC_BM * (XC_BN / 2) + 255) / 256;  // rows per thread upper bound (n >= 1)
+  double rre[MAXR], rim[MAXR];
+#pragma unroll
+  for (int c = 0; c < MAXR; ++c) {
+    const int q = tid + c * 256;
+    double sr = 0.0, si = 0.0;
+    if (q < nrows) {
+      const int b = q / n, i = q - b * n;
+      const int gl = b % gt, wl = b / gt;
+      const double* F = lds + (gl * n) * XC_BN + wl * 2 * n;  // f(i,j) at F[i*XC_BN + 2j]
+      for (int j = 0; j < n; ++j) {
+        const double ar = F[i * XC_BN + 2 * j], ai = F[i * XC_BN + 2 * j + 1];
+        const double br = F[j * XC_BN + 2 * i], bi = F[j * XC_BN + 2 * i + 1];
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
+      }
+    }
+    rre[c] = sr;
+    rim[c] = si;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < MAXR; ++c) {
+    const int q = tid + c * 256;
+    if (q < nrows) {
+      lds[2 * q] = rre[c];
+      lds[2 * q + 1] = rim[c];
+    }
+  }
+  __syncthreads();
+  // one thread per walker of the tile sums its rows (all g blocks of the tile)
+  if (tid < wt && w0 + tid < nw) {
+    const int wl = tid;
+    double sr = 0.0, si = 0.0;
+    for (int gl = 0; gl < gt; ++gl) {
+      if (g0 + gl >= GE) break;
+      const int b = wl * gt + gl;
+      for (int i = 0; i < n; ++i) {
+        sr += lds[2 * (b * n + i)];
+        si += lds[2 * (b * n + i) + 1];
+      }
+    }
+    const int w = sel ? sel[w0 + wl] : w0 + wl;
+    double* o = Xp + 2 * (((int64_t)blockIdx.z * W + w) * ngt + blockIdx.x);
+    o[0] = sr;
+    o[1] = si;
+  }
+}
+
+void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st) {
+  const int n = s.slot[0].n;  // both slots launched in one grid need the same tile shape
+  if (s.nslots == 1 || s.slot[1].n == n) {
+    const int gt = exchange_tile_g(n), wt = exchange_tile_w(n);
+    const int ngt = (int)cdivu(s.GE, gt);
+    dim3 grid(ngt, cdivu(nsel, wt), s.nslots);
+    hipLaunchKernelGGL(k_exchange, grid, dim3(256), 0, st, s.R, s.theta, sel, s.Xp, s.N, s.GE, nsel, s.W,
+                       s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ngt_cap);
+  } else {
+    for (int z = 0; z < s.nslots; ++z) {
+      const int nz = s.slot[z].n;
+      const int gt = exchange_tile_g(nz), wt = exchange_tile_w(nz);
+      const int ngt = (int)cdivu(s.GE, gt);
+      // a one-slot grid whose z index is the slot: pass the slot as s0 and offset Xp
+      dim3 grid(ngt, cdivu(nsel, wt), 1);
+      hipLaunchKernelGGL(k_exchange, grid, dim3(256), 0, st, s.R, s.theta, sel,
+                         s.Xp + 2 * (int64_t)z * s.W * s.ngt_cap, s.N, s.GE, nsel, s.W, s.Ktot, s.slot[z],
+                         s.slot[z], s.ngt_cap);
+    }
+  }
+}
+
+// E_L = ecore + sum_s m_s e1_s + 1/2 sum_{g<GE} fb_g^2 - 1/2 sum_s m_s X_s
+__global__ __launch_bounds__(64) void k_energy(const double* fb, const double* e1, const double* Xp,
+                                               const double* logO, const double* weights, const int* sel,
+                                               double* eloc, double* ho, int G, int GE, int W, int nslots,
+                                               int m0, int m1, int ngt_used0, int ngt_used1, int ngt_cap,
+                                               double ecore, double delta) {
+  const int idx = blockIdx.x;
+  const int w = sel ? sel[idx] : idx;
+  const int lane = threadIdx.x;
+  double jr = 0.0, ji = 0.0;
+  for (int g = lane; g < GE; g += 64) {
+    const double a = fb[(int64_t)w * 2 * G + 2 * g], b = fb[(int64_t)w * 2 * G + 2 * g + 1];
+    jr += a * a - b * b;
+    ji += 2.0 * a * b;
+  }
+  double xr = 0.0, xi = 0.0;
+  for (int s = 0; s < nslots; ++s) {
+    const int m = s == 0 ? m0 : m1, nt = s == 0 ? ngt_used0 : ngt_used1;
+    for (int t = lane; t < nt; t += 64) {
+      const double* o = Xp + 2 * (((int64_t)s * W + w) * ngt_cap + t);
+      xr += m * o[0];
+      xi += m * o[1];
+    }
+  }
+  jr = wave_sum(jr); ji = wave_sum(ji); xr = wave_sum(xr); xi = wave_sum(xi);
+  if (lane == 0) {
+    double er = ecore, ei = 0.0;
+    for (int s = 0; s < nslots; ++s) {
+      const int m = s == 0 ? m0 : m1;
+      er += m * e1[2 * ((int64_t)w * nslots + s)];
+      ei += m * e1[2 * ((int64_t)w * nslots + s) + 1];
+    }
+    er += 0.5 * (jr - xr);
+    ei += 0.5 * (ji - xi);
+    const bool live = weights == nullptr || weights[w] != 0.0;
+    eloc[w] = live ? er + delta : 0.0;
+    if (ho) {
+      const cplx O = cexp(cplx{logO[2 * w], logO[2 * w + 1]});
+      const cplx EO = cmul(cplx{er, ei}, O);
+      ho[4 * w] = EO.re; ho[4 * w + 1] = EO.im; ho[4 * w + 2] = O.re; ho[4 * w + 3] = O.im;
+    }
+  }
+}
+
+void launch_energy(const DevState& s, const int* sel, int nsel, double delta, hipStream_t st) {
+  const int ng0 = (int)cdivu(s.GE, exchange_tile_g(s.slot[0].n));
+  const int ng1 = s.nslots > 1 ? (int)cdivu(s.GE, exchange_tile_g(s.slot[1].n)) : 0;
+  hipLaunchKernelGGL(k_energy, dim3(nsel), dim3(64), 0, st, s.fb, s.e1, s.Xp, s.logO, s.weights, sel, s.eloc,
+                     s.ho, s.G, s.GE, s.W, s.nslots, s.slot[0].mult, s.nslots > 1 ? s.slot[1].mult : 0, ng0,
+                     ng1, s.ngt_cap, s.ecore, delta);
+}
+
+// ------------------------------------------------------------------ comb
+// sequential cumulative sum, exactly the reference's order (DQMC/MixedEstimator.cpp:595-598)
+__global__ void k_cumsum(const double* w, double* cum, int W) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double c = 0.0;
+    for (int i = 0; i < W; ++i) { c = (i == 0) ? w[0] : c + w[i]; cum[i] = c; }
+  }
+}
+__global__ void k_comb_index(const double* cum, int* idx, double* weights, int W, double zeta) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const double tot = cum[W - 1];
+  const double z = (w + zeta) / W;
+  const double target = z * tot;
+  int lo = 0, hi = W;  // lower_bound
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cum[mid] < target) lo = mid + 1; else hi = mid;
+  }
+  idx[w] = lo < W ? lo : W - 1;
+}
+__global__ void k_comb_weights(const double* cum, double* weights, int W) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < W) weights[w] = cum[W - 1] / W;
+}
+__global__ void k_gather(const int* idx, const double* src, double* dst, int64_t per) {
+  const int w = blockIdx.y;
+  const double* s = src + (int64_t)idx[w] * per;
+  double* d = dst + (int64_t)w * per;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+void launch_comb(DevState& s, double zeta, hipStream_t st) {
+  hipLaunchKernelGGL(k_cumsum, dim3(1), dim3(64), 0, st, s.weights, s.cum, s.W);
+  hipLaunchKernelGGL(k_comb_index, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.cum, s.idx, s.weights, s.W, zeta);
+  auto gather = [&](double*& a, double*& b, int64_t per) {
+    const unsigned bx = (unsigned)std::min<int64_t>(64, (per + 255) / 256);
+    hipLaunchKernelGGL(k_gather, dim3(bx, s.W), dim3(256), 0, st, s.idx, a, b, per);
+    std::swap(a, b);
+  };
+  gather(s.psi, s.psi2, 2 * s.ms);
+  gather(s.theta, s.theta2, 2 * s.Ktot);
+  gather(s.fb, s.fb2, 2 * (int64_t)s.G);
+  gather(s.logO, s.logO2, 2);
+  gather(s.e1, s.e12, 2 * (int64_t)s.nslots);
+  hipLaunchKernelGGL(k_comb_weights, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.cum, s.weights, s.W);
+}
+
+__global__ void k_copy_det0(double* dst, const double* src, int64_t ms, int64_t n0, int64_t off1) {
+  const int w = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n0; i += (int64_t)gridDim.x * blockDim.x)
+    dst[2 * (w * ms + off1) + i] = src[2 * w * ms + i];
+}
+void launch_copy_det0_to_det1(const DevState& s, double* dst, const double* src, hipStream_t st) {
+  const int64_t n0 = (int64_t)s.N * s.det_n[0];
+  hipLaunchKernelGGL(k_copy_det0, dim3(cdivu(2 * n0, 256), s.W), dim3(256), 0, st, dst, src, s.ms, n0,
+                     s.det_off[1]);
+}
+
+// ------------------------------------------------------------------ Hamiltonian setup
+// floattenCholesky layout (DQMC/Hamiltonian.cpp:179-220): Lp[g][i(i+1)/2 + j], j <= i
+__global__ void k_pack_chol(const double* L, double* Lp, int N, int G, int64_t T, int round_f32) {
+  const int g = blockIdx.y;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (int64_t)gridDim.x * blockDim.x) {
+    // invert t = i(i+1)/2 + j
+    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(i + 1) * (i + 2) / 2 <= t) ++i;
+    while ((int64_t)i * (i + 1) / 2 > t) --i;
+    const int j = (int)(t - (int64_t)i * (i + 1) / 2);
+    const double v = L[(int64_t)g * N * N + (int64_t)i * N + j];
+    Lp[(int64_t)g * T + t] = round_f32 ? (double)(float)v : v;
+  }
+}
+void launch_pack_chol(const double* L, double* Lp, int N, int G, int64_t T, bool round_f32, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_chol, dim3((unsigned)std::min<int64_t>(256, (T + 255) / 256), G), dim3(256), 0, st, L,
+                     Lp, N, G, T, (int)round_f32);
+}
+
+// rotateCholesky (DQMC/Hamiltonian.cpp:52-133): R[g][koff + k*n + i] = sum_m Phi(m,i) L_g(m,k)
+constexpr int RC_BM = 64, RC_BN = 128;
+__global__ __launch_bounds__(256) void k_rotate_chol(const double* L, const double* phi, double* R, int N, int n,
+                                                     int G, int64_t Ktot, int64_t koff) {
+  using Eng = MmaF64<RC_BM, RC_BN, 2, 2, true, true>;
+  __shared__ double lds[Eng::LDS_DOUBLES];
+  const int64_t ncol = (int64_t)G * N;
+  const int64_t q0 = (int64_t)blockIdx.x * RC_BN;
+  const int row0 = blockIdx.y * RC_BM;
+  Eng e;
+  e.run([&](int r, int m) { const int i = row0 + r; return (i < n && m < N) ? phi[(int64_t)m * n + i] : 0.0; },
+        [&](int m, int c) {
+          const int64_t q = q0 + c;
+          if (q >= ncol || m >= N) return 0.0;
+          const int64_t g = q / N, k = q - g * N;
+          return L[g * N * N + (int64_t)m * N + k];
+        },
+        N, lds);
+  e.store([&](int r, int c, double v) {
+    const int i = row0 + r;
+    const int64_t q = q0 + c;
+    if (i >= n || q >= ncol) return;
+    const int64_t g = q / N, k = q - g * N;
+    R[g * Ktot + koff + k * n + i] = v;
+  });
+}
+void launch_rotate_chol(const double* L, const double* phi, double* R, int N, int n, int G, int64_t Ktot,
+                        int64_t koff, hipStream_t st) {
+  dim3 grid(cdivu((int64_t)G * N, RC_BN), cdivu(n, RC_BM));
+  hipLaunchKernelGGL(k_rotate_chol, grid, dim3(256), 0, st, L, phi, R, N, n, G, Ktot, koff);
+}
+
+// plain row-major C = A B through the tile engine (parity hook for the MFMA lane layout)
+__global__ __launch_bounds__(256) void k_debug_gemm(const double* A, const double* B, double* C, int M, int N, int K) {
+  using Eng = MmaF64<64, 128, 2, 2, false, true>;
+  __shared__ double lds[Eng::LDS_DOUBLES];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 128;
+  Eng e;
+  e.run([&](int r, int k) { return (r0 + r < M && k < K) ? A[(int64_t)(r0 + r) * K + k] : 0.0; },
+        [&](int k, int c) { return (k < K && c0 + c < N) ? B[(int64_t)k * N + c0 + c] : 0.0; }, K, lds);
+  e.store([&](int r, int c, double v) {
+    if (r0 + r < M && c0 + c < N) C[(int64_t)(r0 + r) * N + c0 + c] = v;
+  });
+}
+void launch_debug_gemm(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t st) {
+  hipLaunchKernelGGL(k_debug_gemm, dim3(cdivu(N, 128), cdivu(M, 64)), dim3(256), 0, st, A, B, C, M, N, K);
+}
+
+__global__ void k_fill(double* p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+void launch_fill(double* p, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+__global__ void k_zero_u64(unsigned long long* p) { if (threadIdx.x == 0) *p = 0ull; }
+void launch_zero_counter(unsigned long long* p, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p);
+}
+
+}  // namespace afq
