@@ -182,9 +182,65 @@ int afq_comb(afq_ctx* ctx, double zeta, double* total_weight);
  * force_bias, fields, vhs, propagate, green, weight, ortho, energy, comb. */
 int afq_stage_times(afq_ctx* ctx, double* ms9, int32_t reset);
 
+/* ---------------------------------------------------------------- driver
+ * calcMixedEstimatorLongProp (DQMC/MixedEstimator.cpp:309-699) on one GPU.
+ * nranks virtual ranks of nwalk walkers each reproduce an `mpirun -np nranks`
+ * run: rank r owns walkers [r*nwalk, (r+1)*nwalk) and, in replay mode, draws
+ * from std::mt19937(seed + r) with one std::normal_distribution per walker slot
+ * (executables/DQMC.cpp:58, DQMC/DQMCWalker.h:18); dead walkers draw nothing;
+ * the comb's uniform comes from rank 0's generator. */
+#define AFQ_RNG_REPLAY 0   /* the reference's libstdc++ stream, drawn on the host */
+#define AFQ_RNG_DEVICE 1   /* counter-based Philox4x32-10 normals drawn on the GPU */
+
+typedef struct afq_run_params {
+  int64_t seed;
+  double dt;
+  int32_t nsteps;        /* steps per energy evaluation / reconfiguration     */
+  int32_t nsweeps;       /* stochasticIter: number of blocks (samples lines)  */
+  int32_t ortho_steps;
+  int32_t burn_iter;
+  int32_t nwalk;         /* per virtual rank                                  */
+  int32_t nranks;
+  int32_t walker_rhf;
+  int32_t ref_compat;
+  double cholesky_threshold;
+  double ene0_guess;     /* 1e10: use the reference energy                   */
+  double weight_cap;     /* <= 0: default                                     */
+  int32_t max_steps;     /* > 0: stop after this many steps                   */
+  int32_t rng_mode;      /* AFQ_RNG_REPLAY | AFQ_RNG_DEVICE                   */
+  int32_t warmup_steps;  /* steps excluded from the timed region              */
+  int32_t prec_fb, prec_vhs, prec_exchange;
+  int32_t energy_mode;   /* AFQ_ENERGY_*                                      */
+  int32_t device;
+} afq_run_params;
+
+typedef struct afq_run_result {
+  double ref_energy;
+  int32_t nchol_ene;
+  double delta;
+  int64_t n_large_deviations;
+  double prop_time, ene_time;  /* host wall seconds, as DQMC/MixedEstimator.cpp:664-676 */
+  int64_t timed_steps;
+  double timed_wall_s;         /* wall seconds of the steps after warmup_steps     */
+  double stage_ms[9];          /* device time per stage (afq_stage_times order)     */
+} afq_run_result;
+
+/* ref_up/ref_dn: reference determinant (rhf.txt or uhf.txt by walker type,
+ * DQMC/MixedEstimator.cpp:338-351); init_up/init_dn: the right state every
+ * walker starts from.  weights_out / energies_out: nsweeps entries each (the
+ * samples.dat columns). */
+int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, const double* ref_up,
+                      const double* ref_dn, const double* init_up, const double* init_dn,
+                      const afq_run_params* p, double* weights_out, double* energies_out,
+                      afq_run_result* res);
+
 /* Parity hook for the FP64 MFMA tile engine every contraction above uses:
  * C = A B for host row-major A (M x K), B (K x N); C (M x N). */
 int afq_debug_gemm_f64(const double* A, const double* B, double* C, int32_t M, int32_t N, int32_t K);
+
+/* Streams `bytes` through a plain copy kernel (k_copy_calib: reads and writes
+ * bytes each); a known-size reference for HBM traffic counters. */
+int afq_debug_copy(int64_t bytes);
 
 /* Synchronise the context's stream. */
 int afq_sync(afq_ctx* ctx);

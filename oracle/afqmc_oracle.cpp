@@ -574,12 +574,15 @@ orc_ctx* orc_create(const afq_ham* ham, const afq_trial* left, int walker_rhf, i
     const int n = c->phi[s].c;
     c->R[s].assign((size_t)c->G * n * N, 0.0);
     for (int g = 0; g < c->G; ++g)
-      for (int k = 0; k < N; ++k)
-        for (int i = 0; i < n; ++i) {
-          double acc = 0.0;
-          for (int m = 0; m < N; ++m) acc += c->phi[s](m, i) * c->Lval(g, m, k);
-          c->R[s][(size_t)g * n * N + (size_t)k * n + i] = acc;
+      for (int m = 0; m < N; ++m) {
+        const double* prow = &c->phiT[s].a[0];  // phiT(i, m) at i + m*n
+        for (int k = 0; k < N; ++k) {
+          const double l = c->Lval(g, m, k);
+          if (l == 0.0) continue;
+          double* r = &c->R[s][(size_t)g * n * N + (size_t)k * n];
+          for (int i = 0; i < n; ++i) r[i] += prow[i + (size_t)m * n] * l;
         }
+      }
   }
   build_slots(c);
   c->walkers.resize(nwalk);

@@ -50,6 +50,7 @@ def lib():
         L.afq_stage_times.argtypes = [vp, DP, i32]
         L.afq_sync.argtypes = [vp]
         L.afq_debug_gemm_f64.argtypes = [DP, DP, DP, i32, i32, i32]
+        L.afq_debug_copy.argtypes = [i64]
         _lib = L
     return _lib
 
@@ -72,6 +73,12 @@ def debug_gemm_f64(A, B):
     if rc != 0:
         raise AfqError("debug_gemm: %s" % lib().afq_last_error(None).decode())
     return Cm
+
+
+def debug_copy(nbytes):
+    """Run the plain copy calibration kernel over nbytes (read) + nbytes (write)."""
+    if lib().afq_debug_copy(int(nbytes)) != 0:
+        raise AfqError("debug_copy: %s" % lib().afq_last_error(None).decode())
 
 
 class Context:
@@ -187,3 +194,53 @@ class Context:
 
     def sync(self):
         self._chk(lib().afq_sync(self.ptr), "afq_sync")
+
+
+AFQ_RNG_REPLAY = 0
+AFQ_RNG_DEVICE = 1
+
+
+class RunParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_int64), ("dt", C.c_double), ("nsteps", C.c_int32), ("nsweeps", C.c_int32),
+        ("ortho_steps", C.c_int32), ("burn_iter", C.c_int32), ("nwalk", C.c_int32),
+        ("nranks", C.c_int32), ("walker_rhf", C.c_int32), ("ref_compat", C.c_int32),
+        ("cholesky_threshold", C.c_double), ("ene0_guess", C.c_double), ("weight_cap", C.c_double),
+        ("max_steps", C.c_int32), ("rng_mode", C.c_int32), ("warmup_steps", C.c_int32),
+        ("prec_fb", C.c_int32), ("prec_vhs", C.c_int32), ("prec_exchange", C.c_int32),
+        ("energy_mode", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class RunResult(C.Structure):
+    _fields_ = [("ref_energy", C.c_double), ("nchol_ene", C.c_int32), ("delta", C.c_double),
+                ("n_large_deviations", C.c_int64), ("prop_time", C.c_double),
+                ("ene_time", C.c_double), ("timed_steps", C.c_int64), ("timed_wall_s", C.c_double),
+                ("stage_ms", C.c_double * 9)]
+
+
+def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, nsweeps, ortho_steps,
+                  nwalk, nranks=1, walker_rhf, ref_compat=False, cholesky_threshold=1e-3,
+                  burn_iter=0, ene0_guess=1e10, weight_cap=0.0, max_steps=0,
+                  rng_mode=AFQ_RNG_REPLAY, warmup_steps=0, prec_fb=AFQ_PREC_FP64,
+                  prec_vhs=AFQ_PREC_FP64, prec_exchange=AFQ_PREC_FP64,
+                  energy_mode=AFQ_ENERGY_EXACT, device=0):
+    """calcMixedEstimatorLongProp on the GPU; returns (weights, energies, RunResult)."""
+    L = lib()
+    if not hasattr(L, "_run_bound"):
+        L.afq_run_long_prop.argtypes = [C.POINTER(AfqHam), C.POINTER(AfqTrial), DP, DP, DP, DP,
+                                        C.POINTER(RunParams), DP, DP, C.POINTER(RunResult)]
+        L._run_bound = True
+    hs, ts = ham.ham_struct(), ham.trial_struct()
+    p = RunParams(seed, dt, nsteps, nsweeps, ortho_steps, burn_iter, nwalk, nranks, int(walker_rhf),
+                  int(ref_compat), cholesky_threshold, ene0_guess, weight_cap, max_steps, rng_mode,
+                  warmup_steps, prec_fb, prec_vhs, prec_exchange, energy_mode, device)
+    w = np.zeros(nsweeps)
+    e = np.zeros(nsweeps)
+    res = RunResult()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (ref_up, ref_dn, init_up, init_dn)]
+    rc = L.afq_run_long_prop(C.byref(hs), C.byref(ts), *[dptr(a) for a in arrs], C.byref(p),
+                             dptr(w), dptr(e), C.byref(res))
+    if rc != 0:
+        raise AfqError("afq_run_long_prop(%d): %s" % (rc, L.afq_last_error(None).decode()))
+    return w, e, res

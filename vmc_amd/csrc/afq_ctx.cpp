@@ -571,13 +571,13 @@ int afq_measure(afq_ctx* c, int32_t mode, double delta, double* eloc, double* ha
   if (mode != AFQ_ENERGY_EXACT) return fail(c, AFQ_ERR_UNSUPPORTED, "energy mode not available");
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
-  {
-    Stage t(c, afq::ST_ENERGY);
-    if (s.GE > 0) afq::launch_exchange(s, nullptr, s.W, c->stream);
+  if (s.GE > 0) {
+    Stage t(c, afq::ST_ENERGY);  // the exchange contraction alone
+    afq::launch_exchange(s, nullptr, s.W, c->stream);
     launch_check("exchange");
-    afq::launch_energy(s, nullptr, s.W, delta, c->stream);
-    launch_check("energy");
   }
+  afq::launch_energy(s, nullptr, s.W, delta, c->stream);
+  launch_check("energy");
   if (eloc) download(eloc, s.eloc, s.W, c->stream);
   if (ham_ovlp) download(ham_ovlp, s.ho, 4 * (size_t)s.W, c->stream);
   return AFQ_OK;
@@ -665,6 +665,23 @@ int afq_debug_gemm_f64(const double* A, const double* B, double* C, int32_t M, i
     launch_check("debug_gemm");
     download(C, dC, (size_t)M * N, nullptr);
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);
+  } catch (const DevError& e) {
+    return fail(nullptr, e.code, e.what());
+  }
+  return AFQ_OK;
+}
+
+int afq_debug_copy(int64_t bytes) {
+  if (bytes <= 0) return fail(nullptr, AFQ_ERR_ARG, "bad size");
+  try {
+    const int64_t n = bytes / 8;
+    double* a = dalloc<double>(n);
+    double* b = dalloc<double>(n);
+    afq::launch_fill(a, n, 1.0, nullptr);
+    afq::launch_copy_calib(a, b, n, nullptr);
+    launch_check("copy_calib");
+    hip_check(hipDeviceSynchronize(), "sync");
+    (void)hipFree(a); (void)hipFree(b);
   } catch (const DevError& e) {
     return fail(nullptr, e.code, e.what());
   }

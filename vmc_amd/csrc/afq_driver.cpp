@@ -1,0 +1,208 @@
+// calcMixedEstimatorLongProp (DQMC/MixedEstimator.cpp:309-699) over the C ABI:
+// reference energy and Cholesky truncation, walker setup, the step loop with
+// population control, periodic re-orthonormalisation, measurement with the
+// E_L clip, and comb reconfiguration.  Single GPU; nranks virtual ranks model
+// an `mpirun -np nranks` run of the reference (rank streams, rank-0 comb,
+// rank-local running averages).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../../include/vmc_afqmc.h"
+
+namespace {
+
+struct CtxGuard {
+  afq_ctx* c = nullptr;
+  ~CtxGuard() {
+    if (c) afq_destroy(c);
+  }
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, const double* ref_up,
+                                 const double* ref_dn, const double* init_up, const double* init_dn,
+                                 const afq_run_params* p, double* weights_out, double* energies_out,
+                                 afq_run_result* res) {
+  if (!ham || !left || !ref_up || !ref_dn || !init_up || !init_dn || !p) return AFQ_ERR_ARG;
+  const int R = p->nranks > 0 ? p->nranks : 1, nw = p->nwalk, W = R * nw;
+  const int N = ham->norb, na = ham->nalpha, nb = ham->nbeta, G = ham->nchol;
+  const double dt = p->dt;
+  afq_params cp{};
+  cp.nwalk = W;
+  cp.walker_rhf = p->walker_rhf;
+  cp.ref_compat = p->ref_compat;
+  cp.prec_fb = p->prec_fb;
+  cp.prec_vhs = p->prec_vhs;
+  cp.prec_exchange = p->prec_exchange;
+  cp.device = p->device;
+  // the reference's default cap uses the per-rank walker count (DQMC/MixedEstimator.cpp:452-454)
+  cp.weight_cap = p->weight_cap > 0 ? p->weight_cap : std::max(100., nw / 10.);
+  CtxGuard g;
+  int rc = afq_create(ham, left, &cp, &g.c);
+  if (rc) return rc;
+  afq_ctx* c = g.c;
+  const size_t ms = (size_t)N * (na + nb);
+  std::vector<double> ref(2 * ms, 0.0), init(2 * ms, 0.0);
+  for (int i = 0; i < N; ++i) {
+    for (int j = 0; j < na; ++j) {
+      ref[2 * ((size_t)i * na + j)] = ref_up[(size_t)i * na + j];
+      init[2 * ((size_t)i * na + j)] = init_up[(size_t)i * na + j];
+    }
+    for (int j = 0; j < nb; ++j) {
+      const size_t q = (size_t)N * na + (size_t)i * nb + j;
+      ref[2 * q] = ref_dn[(size_t)i * nb + j];
+      init[2 * q] = init_dn[(size_t)i * nb + j];
+    }
+  }
+  // reference energy and Cholesky truncation for E_L (:352-384)
+  double o4[4];
+  if ((rc = afq_ham_and_overlap(c, ref.data(), 1, G, o4))) return rc;
+  auto cdivc = [](double ar, double ai, double br, double bi, double& cr, double& ci) {
+    const double d = br * br + bi * bi;
+    cr = (ar * br + ai * bi) / d;
+    ci = (ai * br - ar * bi) / d;
+  };
+  double ref_re, ref_im;
+  cdivc(o4[0], o4[1], o4[2], o4[3], ref_re, ref_im);
+  const double ene0 = (p->ene0_guess == 1.e10) ? ref_re : p->ene0_guess;
+  int nchol = G;
+  double delta = 0.0;
+  const int cand[5] = {int(0.3 * G), int(0.4 * G), int(0.5 * G), int(0.6 * G), int(0.7 * G)};
+  for (int i = 0; i < 5; ++i) {
+    if ((rc = afq_ham_and_overlap(c, ref.data(), 1, cand[i], o4))) return rc;
+    double er, ei;
+    cdivc(o4[0], o4[1], o4[2], o4[3], er, ei);
+    if (std::hypot(ref_re - er, ref_im - ei) < p->cholesky_threshold) {
+      nchol = cand[i];
+      delta = ref_re - er;
+      break;
+    }
+  }
+  if ((rc = afq_set_nchol_ene(c, nchol))) return rc;
+  if ((rc = afq_prep_prop(c, ref_up, ref_dn, nullptr, nullptr, dt, ene0))) return rc;
+  std::vector<double> dets((size_t)W * 2 * ms);
+  for (int w = 0; w < W; ++w) std::memcpy(&dets[(size_t)w * 2 * ms], init.data(), 2 * ms * sizeof(double));
+  if ((rc = afq_set_walkers(c, dets.data(), nullptr, nullptr))) return rc;
+  dets.clear();
+  dets.shrink_to_fit();
+  std::vector<double> eloc(W), weights(W, 1.0);
+  const int nsweeps = p->nsweeps, nsteps = p->nsteps;
+  std::vector<double> totalWeights(nsweeps, 0.0), totalEnergies(nsweeps, 0.0);
+  if ((rc = afq_measure(c, AFQ_ENERGY_EXACT, 0.0, eloc.data(), nullptr))) return rc;
+  double esum = 0.0;
+  for (int w = 0; w < W; ++w) esum += eloc[w];
+  totalWeights[0] = W;
+  totalEnergies[0] = esum / W + delta;
+  // random streams
+  std::vector<std::mt19937> gens;
+  for (int r = 0; r < R; ++r) gens.emplace_back((unsigned)(p->seed + r));
+  std::vector<std::normal_distribution<double>> normals(W, std::normal_distribution<double>(0., 1.));
+  std::vector<double> draws;
+  if (p->rng_mode == AFQ_RNG_REPLAY) draws.assign((size_t)W * G, 0.0);
+  double eshift = totalEnergies[0];
+  double totalWeight = W;
+  std::vector<double> averageEnergy(R, totalEnergies[0]), averageEnergyEql(R, totalEnergies[0]);
+  double averageNum = 0., averageDenom = 0., averageNumEql = 0., averageDenomEql = 0.;
+  double eEstimate = totalEnergies[0];
+  int64_t nLarge = 0;
+  double propTime = 0., eneTime = 0., tStart = 0.;
+  int64_t timed = 0;
+  afq_stage_times(c, nullptr, 1);
+  const int64_t last = (int64_t)nsweeps * nsteps;
+  for (int64_t step = 1; step < last; ++step) {
+    if (p->max_steps > 0 && step > p->max_steps) break;
+    if (step == p->warmup_steps + 1) {
+      afq_sync(c);
+      afq_stage_times(c, nullptr, 1);
+      tStart = now_s();
+    }
+    for (int r = 0; r < R; ++r)
+      if (step * dt < 10.) averageEnergy[r] = averageEnergyEql[r];
+    double t0 = now_s();
+    int64_t ncap = 0;
+    if (p->rng_mode == AFQ_RNG_REPLAY) {
+      for (int r = 0; r < R; ++r)
+        for (int wl = 0; wl < nw; ++wl) {
+          const int w = r * nw + wl;
+          if (weights[w] > 1.e-8)
+            for (int q = 0; q < G; ++q) draws[(size_t)w * G + q] = normals[w](gens[r]);
+        }
+      rc = afq_propagate(c, draws.data(), 0, (uint64_t)step, eshift, nullptr, &ncap);
+    } else {
+      rc = afq_propagate(c, nullptr, (uint64_t)p->seed, (uint64_t)step, eshift, nullptr, &ncap);
+    }
+    if (rc) return rc;
+    nLarge += ncap;
+    if ((rc = afq_get_weights(c, weights.data()))) return rc;
+    propTime += now_s() - t0;
+    totalWeight = 0.0;
+    for (int w = 0; w < W; ++w) totalWeight += weights[w];
+    eshift = eEstimate - 0.1 * std::log(totalWeight / (double)W) / dt;
+    if (step % p->ortho_steps == 0)
+      if ((rc = afq_orthogonalize(c))) return rc;
+    if (step % nsteps == 0) {
+      const int block = (int)(step / nsteps);
+      t0 = now_s();
+      if ((rc = afq_measure(c, p->energy_mode, delta, eloc.data(), nullptr))) return rc;
+      eneTime += now_s() - t0;
+      for (int r = 0; r < R; ++r)
+        for (int wl = 0; wl < nw; ++wl) {
+          const int w = r * nw + wl;
+          if (weights[w] == 0.) { eloc[w] = 0.; continue; }
+          if (std::isnan(eloc[w]) || std::isinf(eloc[w])) return AFQ_ERR_NUMERIC;
+          if (std::fabs(eloc[w] - averageEnergy[r]) > std::sqrt(2. / dt)) {
+            ++nLarge;
+            eloc[w] = eloc[w] > averageEnergy[r] ? averageEnergy[r] + std::sqrt(2. / dt)
+                                                 : averageEnergy[r] - std::sqrt(2. / dt);
+          }
+        }
+      double we = 0.0;
+      for (int w = 0; w < W; ++w) we += eloc[w] * weights[w];
+      totalWeights[block] = totalWeight;
+      totalEnergies[block] = we / totalWeight;
+      if (step * dt < 10. || step < (int64_t)p->burn_iter * nsteps) {
+        averageNumEql += totalEnergies[block] * totalWeights[block];
+        averageDenomEql += totalWeights[block];
+        averageEnergyEql[0] = averageNumEql / averageDenomEql;
+      } else {
+        averageNum += totalEnergies[block] * totalWeights[block];
+        averageDenom += totalWeights[block];
+        averageEnergy[0] = averageNum / averageDenom;
+      }
+      eEstimate = 0.9 * eEstimate + 0.1 * totalEnergies[block];
+      std::uniform_real_distribution<double> uniform(0., 1.);
+      const double zeta = uniform(gens[0]);
+      double tot = 0.0;
+      if ((rc = afq_comb(c, zeta, &tot))) return rc;
+      totalWeight = tot;
+      for (int w = 0; w < W; ++w) weights[w] = tot / W;
+    }
+    if (step > p->warmup_steps) ++timed;
+  }
+  afq_sync(c);
+  const double tEnd = now_s();
+  for (int i = 0; i < nsweeps; ++i) {
+    if (weights_out) weights_out[i] = totalWeights[i];
+    if (energies_out) energies_out[i] = totalEnergies[i];
+  }
+  if (res) {
+    res->ref_energy = ref_re;
+    res->nchol_ene = nchol;
+    res->delta = delta;
+    res->n_large_deviations = nLarge;
+    res->prop_time = propTime;
+    res->ene_time = eneTime;
+    res->timed_steps = timed;
+    res->timed_wall_s = timed > 0 ? tEnd - tStart : 0.0;
+    afq_stage_times(c, res->stage_ms, 1);
+  }
+  return AFQ_OK;
+}

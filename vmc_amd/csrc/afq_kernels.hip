@@ -905,6 +905,15 @@ void launch_debug_gemm(const double* A, const double* B, double* C, int M, int N
   hipLaunchKernelGGL(k_debug_gemm, dim3(cdivu(N, 128), cdivu(M, 64)), dim3(256), 0, st, A, B, C, M, N, K);
 }
 
+// plain streaming copy (read n, write n doubles): the HBM-traffic calibration kernel
+__global__ void k_copy_calib(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+void launch_copy_calib(const double* a, double* b, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_calib, dim3(2048), dim3(256), 0, st, a, b, n);
+}
+
 __global__ void k_fill(double* p, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
