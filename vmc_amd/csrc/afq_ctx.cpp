@@ -175,7 +175,7 @@ void alloc_walkers(DevState& s) {
   s.factors = dalloc<double>(W);
   s.fields = dalloc<double>(2 * W * s.G);
   s.sft = dalloc<double>(4 * W);
-  s.P = dalloc<double>(2 * W * s.T);
+  s.P = dalloc<double>(2 * W * (int64_t)s.N * s.N);
   s.t0 = dalloc<double>(2 * W * s.ms);
   s.t1 = dalloc<double>(2 * W * s.ms);
   s.accb = dalloc<double>(2 * W * s.ms);

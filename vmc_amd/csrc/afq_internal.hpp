@@ -63,7 +63,7 @@ struct DevState {
   // step scratch
   double* fields = nullptr;  // W x G complex
   double* sft = nullptr;     // W x 4: shift (complex), fbTerm (complex)
-  double* P = nullptr;       // W x 2 x T (VHS: sum_g Re/Im(field) L_g)
+  double* P = nullptr;       // W x 2 x N x N (VHS: sum_g Re/Im(field) L_g, both triangles)
   double* t0 = nullptr; double* t1 = nullptr; double* accb = nullptr;  // W x ms complex
   double* draws = nullptr;   // W x G (replay)
   double* Xp = nullptr;      // nslots x W x ngt complex exchange partials

@@ -42,7 +42,7 @@ struct FbB {  // B(k, g) = R[g][k]
 };
 
 constexpr int FB_BM = 128, FB_BN = 128;
-__global__ __launch_bounds__(256) void k_force_bias(const double* R, const double* th, double* fb,
+__global__ __launch_bounds__(256, 2) void k_force_bias(const double* R, const double* th, double* fb,
                                                     int G, int W, int64_t Ktot, double alpha) {
   using Eng = MmaF64<FB_BM, FB_BN, 2, 2, false, false>;
   __shared__ double lds[Eng::LDS_DOUBLES];
@@ -145,8 +145,8 @@ struct VhsB {  // B(g, t) = Lp[g][t]
   }
 };
 constexpr int VHS_BM = 128, VHS_BN = 128;
-__global__ __launch_bounds__(256) void k_vhs(const double* fields, const double* Lp, double* P,
-                                             const double* weights, int G, int W, int64_t T,
+__global__ __launch_bounds__(256, 2) void k_vhs(const double* fields, const double* Lp, double* P,
+                                             const double* weights, int G, int W, int64_t T, int N,
                                              int ref_compat) {
   using Eng = MmaF64<VHS_BM, VHS_BN, 2, 2, false, true>;
   __shared__ double lds[Eng::LDS_DOUBLES];
@@ -161,13 +161,22 @@ __global__ __launch_bounds__(256) void k_vhs(const double* fields, const double*
   e.store([&](int r, int c, double v) {
     const int q = row0 + r;
     const int64_t t = t0 + c;
-    if (q < 2 * W && t < T) P[(int64_t)q * T + t] = ref_compat ? (double)(float)v : v;
+    if (q >= 2 * W || t >= T) return;
+    // packed t = i(i+1)/2 + j (j <= i) -> both triangles of the N x N matrix
+    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    if ((int64_t)(i + 1) * (i + 2) / 2 <= t) ++i;
+    if ((int64_t)i * (i + 1) / 2 > t) --i;
+    const int j = (int)(t - (int64_t)i * (i + 1) / 2);
+    const double x = ref_compat ? (double)(float)v : v;
+    double* Pq = P + (int64_t)q * N * N;
+    Pq[(int64_t)i * N + j] = x;
+    Pq[(int64_t)j * N + i] = x;
   });
 }
 
 void launch_vhs(const DevState& s, hipStream_t st) {
   dim3 grid(cdivu(s.T, VHS_BN), cdivu(2 * (int64_t)s.W, VHS_BM));
-  hipLaunchKernelGGL(k_vhs, grid, dim3(256), 0, st, s.fields, s.Lp, s.P, s.weights, s.G, s.W, s.T,
+  hipLaunchKernelGGL(k_vhs, grid, dim3(256), 0, st, s.fields, s.Lp, s.P, s.weights, s.G, s.W, s.T, s.N,
                      (int)s.ref_compat);
 }
 
@@ -212,15 +221,12 @@ __global__ __launch_bounds__(256) void k_expmul(const double* E, const double* s
 }
 
 // t_dst = (V t_src)/p, acc += t_dst for one walker per blockIdx.z; V = i sqrt(dt) (P_re + i P_im)
-struct VA {  // [A_re | A_im], A_re = -sq P_im, A_im = sq P_re, symmetric packed
+struct VA {  // [A_re | A_im], A_re = -sq P_im, A_im = sq P_re (full N x N rows)
   const double* Pre; const double* Pim; int N; int row0; double sq;
   __device__ double operator()(int r, int k) const {
     const int i = row0 + r;
     if (i >= N || k >= 2 * N) return 0.0;
-    const int kk = k < N ? k : k - N;
-    const int a = i > kk ? i : kk, b = i > kk ? kk : i;
-    const int64_t idx = (int64_t)a * (a + 1) / 2 + b;
-    return k < N ? -sq * Pim[idx] : sq * Pre[idx];
+    return k < N ? -sq * Pim[(int64_t)i * N + k] : sq * Pre[(int64_t)i * N + (k - N)];
   }
 };
 struct TB {  // [[B_re, B_im], [-B_im, B_re]] over the walker's propagated dets, interleaved cols
@@ -249,8 +255,8 @@ __global__ __launch_bounds__(256) void k_vmul(const double* P, const double* tsr
   if (weights[w] <= 1e-8) return;
   const int row0 = blockIdx.y * VM_BM, q0 = blockIdx.x * VM_BN;
   const int ncol = n2a + n2b;
-  const double* Pre = P + (int64_t)(2 * w) * T;
-  const double* Pim = Pre + T;
+  const double* Pre = P + (int64_t)(2 * w) * N * N;
+  const double* Pim = Pre + (int64_t)N * N;
   const double* src = tsrc + (int64_t)w * wstride;
   Eng e;
   e.run(VA{Pre, Pim, N, row0, sq}, TB{src, N, n2a, n2b, offb, q0, ncol}, 2 * N, lds);
@@ -636,12 +642,15 @@ struct XcB {  // B(k, (w,j,c)) = theta_s[w][k][j][c]
   }
 };
 
-// Xp[s][w][gtile] = sum_{g in tile} sum_ij f_ij f_ji,  f = R_{s,g} theta_s
-__global__ __launch_bounds__(256) void k_exchange(const double* R, const double* theta, const int* sel,
-                                                  double* Xp, int N, int GE, int nw, int W, int64_t Ktot,
-                                                  Slot s0, Slot s1, int ngt) {
+// Xp[s][w][gtile] = sum_{g in tile} sum_ij f_ij f_ji,  f = R_{s,g} theta_s.
+// The output tile is reduced one g-block at a time: its n rows are staged in
+// LDS (<= 64 x XC_BN doubles), thread (wl, i) sums row i of walker wl's block
+// and keeps a running total, so f never leaves the workgroup.
+__global__ __launch_bounds__(256, 2) void k_exchange(const double* R, const double* theta, const int* sel,
+                                                     double* Xp, int N, int GE, int nw, int W, int64_t Ktot,
+                                                     Slot s0, Slot s1, int ngt) {
   using Eng = MmaF64<XC_BM, XC_BN, 2, 2, true, true>;
-  constexpr int FT = XC_BM * XC_BN;
+  constexpr int FT = (XC_BN / 2) * XC_BN;
   constexpr int LDSN = Eng::LDS_DOUBLES > FT ? Eng::LDS_DOUBLES : FT;
   __shared__ double lds[LDSN];
   const Slot sl = blockIdx.z == 0 ? s0 : s1;
@@ -649,55 +658,42 @@ __global__ __launch_bounds__(256) void k_exchange(const double* R, const double*
   const int g0 = blockIdx.x * gt, w0 = blockIdx.y * wt;
   Eng e;
   e.run(XcA{R, Ktot, sl.koff, n, gt, g0, GE, N}, XcB{theta, sel, Ktot, sl.koff, n, wt, w0, nw, N}, N, lds);
-  // stage the output tile (row-major XC_BM x XC_BN) in LDS
-  e.store([&](int r, int c, double v) { lds[r * XC_BN + c] = v; });
-  __syncthreads();
-  // per (block, row i): sum_j f_ij f_ji (complex)
-  const int nrows = gt * wt * n;
-  const int tid = threadIdx.x;
-  constexpr int MAXR = (XC_BM * (XC_BN / 2) + 255) / 256;  // rows per thread upper bound (n >= 1)
-  double rre[MAXR], rim[MAXR];
-#pragma unroll
-  for (int c = 0; c < MAXR; ++c) {
-    const int q = tid + c * 256;
-    double sr = 0.0, si = 0.0;
-    if (q < nrows) {
-      const int b = q / n, i = q - b * n;
-      const int gl = b % gt, wl = b / gt;
-      const double* F = lds + (gl * n) * XC_BN + wl * 2 * n;  // f(i,j) at F[i*XC_BN + 2j]
+  const int tid = threadIdx.x, nr = wt * n;
+  double accr = 0.0, acci = 0.0;
+  for (int gl = 0; gl < gt; ++gl) {
+    if (g0 + gl >= GE) break;
+    e.store([&](int r, int c, double v) {
+      const int rr = r - gl * n;
+      if (rr >= 0 && rr < n) lds[rr * XC_BN + c] = v;
+    });
+    __syncthreads();
+    if (tid < nr) {
+      const int wl = tid / n, i = tid - wl * n;
+      const double* F = lds + wl * 2 * n;  // f(i,j) at F[i*XC_BN + 2j]
+      double sr = 0.0, si = 0.0;
       for (int j = 0; j < n; ++j) {
         const double ar = F[i * XC_BN + 2 * j], ai = F[i * XC_BN + 2 * j + 1];
         const double br = F[j * XC_BN + 2 * i], bi = F[j * XC_BN + 2 * i + 1];
         sr += ar * br - ai * bi;
         si += ar * bi + ai * br;
       }
+      accr += sr;
+      acci += si;
     }
-    rre[c] = sr;
-    rim[c] = si;
+    __syncthreads();
+  }
+  if (tid < nr) {
+    lds[2 * tid] = accr;
+    lds[2 * tid + 1] = acci;
   }
   __syncthreads();
-#pragma unroll
-  for (int c = 0; c < MAXR; ++c) {
-    const int q = tid + c * 256;
-    if (q < nrows) {
-      lds[2 * q] = rre[c];
-      lds[2 * q + 1] = rim[c];
-    }
-  }
-  __syncthreads();
-  // one thread per walker of the tile sums its rows (all g blocks of the tile)
   if (tid < wt && w0 + tid < nw) {
-    const int wl = tid;
     double sr = 0.0, si = 0.0;
-    for (int gl = 0; gl < gt; ++gl) {
-      if (g0 + gl >= GE) break;
-      const int b = wl * gt + gl;
-      for (int i = 0; i < n; ++i) {
-        sr += lds[2 * (b * n + i)];
-        si += lds[2 * (b * n + i) + 1];
-      }
+    for (int i = 0; i < n; ++i) {
+      sr += lds[2 * (tid * n + i)];
+      si += lds[2 * (tid * n + i) + 1];
     }
-    const int w = sel ? sel[w0 + wl] : w0 + wl;
+    const int w = sel ? sel[w0 + tid] : w0 + tid;
     double* o = Xp + 2 * (((int64_t)blockIdx.z * W + w) * ngt + blockIdx.x);
     o[0] = sr;
     o[1] = si;
