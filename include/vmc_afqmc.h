@@ -238,6 +238,12 @@ int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, const double* r
  * C = A B for host row-major A (M x K), B (K x N); C (M x N). */
 int afq_debug_gemm_f64(const double* A, const double* B, double* C, int32_t M, int32_t N, int32_t K);
 
+/* Parity hook for the BF16 MFMA tile engine: A, B as above (FP64, split on the
+ * device into bf16 hi/lo), C = A B accumulated in FP32 with passes = 3
+ * (hi*hi + hi*lo + lo*hi) or 1 (hi*hi). */
+int afq_debug_gemm_bf16(const double* A, const double* B, float* C, int32_t M, int32_t N, int32_t K,
+                        int32_t passes);
+
 /* Streams `bytes` through a plain copy kernel (k_copy_calib: reads and writes
  * bytes each); a known-size reference for HBM traffic counters. */
 int afq_debug_copy(int64_t bytes);

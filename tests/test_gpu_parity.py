@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from vmc_amd.afqmc import Context, debug_gemm_f64
+from vmc_amd.afqmc import Context, debug_gemm_bf16, debug_gemm_f64
 from vmc_amd.formats import load_case
 from vmc_amd.synthetic import make_system
 
@@ -40,6 +40,30 @@ def test_mfma_engine_layout():
         B = rng.standard_normal((K, N))
         Cg = debug_gemm_f64(A, B)
         assert rel(Cg, A @ B) < 1e-13, (M, K, N)
+
+
+def bf16_round(x):
+    """float64 -> nearest-even bfloat16 (through float32), returned as float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("passes", [1, 3])
+def test_bf16_engine_layout_and_split(passes):
+    """BF16 engine == the exact product of the bf16 hi/lo operands (FP32 accumulation
+    only), and BF16x3 is within 1e-5 of the FP64 product."""
+    rng = np.random.default_rng(1)
+    for M, K, N in [(16, 32, 16), (70, 300, 150), (130, 45, 257)]:
+        A = rng.standard_normal((M, K))
+        B = rng.standard_normal((K, N))
+        Cg = debug_gemm_bf16(A, B, passes).astype(np.float64)
+        Ah, Bh = bf16_round(A), bf16_round(B)
+        Al, Bl = bf16_round(A - Ah), bf16_round(B - Bh)
+        ref = Ah @ Bh + ((Ah @ Bl + Al @ Bh) if passes == 3 else 0.0)
+        assert rel(Cg, ref) < 2e-6, (M, K, N, rel(Cg, ref))
+        if passes == 3:
+            assert rel(Cg, A @ B) < 1e-5
 
 
 def perturbed_walkers(h, W, walker_rhf, rng, scale=0.25):

@@ -51,6 +51,7 @@ def lib():
         L.afq_sync.argtypes = [vp]
         L.afq_debug_gemm_f64.argtypes = [DP, DP, DP, i32, i32, i32]
         L.afq_debug_copy.argtypes = [i64]
+        L.afq_debug_gemm_bf16.argtypes = [DP, DP, C.POINTER(C.c_float), i32, i32, i32, i32]
         _lib = L
     return _lib
 
@@ -72,6 +73,18 @@ def debug_gemm_f64(A, B):
     rc = lib().afq_debug_gemm_f64(dptr(A), dptr(B), dptr(Cm), M, N, K)
     if rc != 0:
         raise AfqError("debug_gemm: %s" % lib().afq_last_error(None).decode())
+    return Cm
+
+
+def debug_gemm_bf16(A, B, passes=3):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    M, K = A.shape
+    N = B.shape[1]
+    Cm = np.zeros((M, N), np.float32)
+    rc = lib().afq_debug_gemm_bf16(dptr(A), dptr(B), Cm.ctypes.data_as(C.POINTER(C.c_float)), M, N, K, passes)
+    if rc != 0:
+        raise AfqError("debug_gemm_bf16: %s" % lib().afq_last_error(None).decode())
     return Cm
 
 

@@ -671,6 +671,28 @@ int afq_debug_gemm_f64(const double* A, const double* B, double* C, int32_t M, i
   return AFQ_OK;
 }
 
+int afq_debug_gemm_bf16(const double* A, const double* B, float* C, int32_t M, int32_t N, int32_t K,
+                        int32_t passes) {
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0 || (passes != 1 && passes != 3))
+    return fail(nullptr, AFQ_ERR_ARG, "bad gemm arguments");
+  try {
+    const int64_t Kp = (K + 7) / 8 * 8;
+    double* dA = dalloc<double>((size_t)M * K);
+    double* dB = dalloc<double>((size_t)K * N);
+    float* dC = dalloc<float>((size_t)M * N);
+    void* scr = dalloc<uint16_t>((size_t)2 * (M + N) * Kp);
+    upload(dA, A, (size_t)M * K, nullptr);
+    upload(dB, B, (size_t)K * N, nullptr);
+    afq::launch_debug_gemm_bf16(dA, dB, dC, M, N, K, passes, scr, nullptr);
+    launch_check("debug_gemm_bf16");
+    hip_check(hipMemcpy(C, dC, (size_t)M * N * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(scr);
+  } catch (const DevError& e) {
+    return fail(nullptr, e.code, e.what());
+  }
+  return AFQ_OK;
+}
+
 int afq_debug_copy(int64_t bytes) {
   if (bytes <= 0) return fail(nullptr, AFQ_ERR_ARG, "bad size");
   try {

@@ -64,6 +64,38 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
 }
 
 // ---------------------------------------------------------------------------
+// Tile scheduling.  Kernels launch a 1-D grid of nx*ny*nz workgroups; the
+// hardware deals consecutive workgroup ids round-robin over the 8 XCDs (each
+// with its own L2).  tile_id() first makes each XCD's share a contiguous range
+// of logical ids, then walks (x, y) in groups of `gy` rows along y (y fastest
+// inside a group) so that the workgroups resident on one XCD at a time share
+// both their A and their B operand tiles in that XCD's L2; z is slowest.
+// ---------------------------------------------------------------------------
+constexpr unsigned kXCDs = 8;
+struct TileId {
+  int x, y, z;
+};
+__device__ __forceinline__ TileId tile_id(int nx, int ny, int nz, int gy) {
+  const unsigned total = (unsigned)nx * ny * nz;
+  const unsigned b = blockIdx.x;
+  const unsigned per = total / kXCDs;
+  const unsigned pid = (b < per * kXCDs) ? (b % kXCDs) * per + b / kXCDs : b;
+  const unsigned per_z = (unsigned)nx * ny;
+  const int z = (int)(pid / per_z);
+  const unsigned p = pid - (unsigned)z * per_z;
+  const unsigned gsz = (unsigned)gy * nx;
+  const unsigned grp = p / gsz;
+  const int y0 = (int)(grp * gy);
+  const int ysz = min(ny - y0, gy);
+  const unsigned r = p - grp * gsz;
+  TileId t;
+  t.y = y0 + (int)(r % ysz);
+  t.x = (int)(r / ysz);
+  t.z = z;
+  return t;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 MFMA tile engine.
 //
 // v_mfma_f64_16x16x4_f64 operand layout (one 64-lane wave):
@@ -169,6 +201,160 @@ struct MmaF64 {
 
   // tile-relative coordinates of accumulator element acc[i][j][r]
   __device__ __forceinline__ int row(int i, int r) const { return wm * WM + i * 16 + (lane >> 4) + 4 * r; }
+  __device__ __forceinline__ int col(int j) const { return wn * WN + j * 16 + (lane & 15); }
+
+  template <class EP>
+  __device__ __forceinline__ void store(const EP& ep) const {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ep(row(i, r), col(j), acc[i][j][r]);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA tile engine with hi/lo split components.
+//
+// Both operands are K-contiguous in global memory and padded to a multiple of
+// 8 bf16 along K: A as rows [m][k], B as columns [n][k] (i.e. B^T row-major).
+// The loaders return, for a split component c and a tile-relative row/column,
+// a pointer to that row's bf16 data (or nullptr: zero row).  A stage of BK=32
+// is copied global -> registers -> LDS in 16-byte chunks, one stage ahead.
+// With NA = NB = 2 (x = hi + lo) the engine accumulates in FP32
+//     acc += A_hi B_hi + A_hi B_lo + A_lo B_hi      ("BF16x3")
+// and with NA = NB = 1 the single product A_hi B_hi.
+//
+// v_mfma_f32_16x16x32_bf16 operand layout (one 64-lane wave):
+//   A (16x32): lane l supplies A[l % 16][8*(l/16) + 0..7]
+//   B (32x16): lane l supplies B[8*(l/16) + 0..7][l % 16]
+//   D (16x16): lane l holds D[4*(l/16) + r][l % 16] in element r = 0..3
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+
+__device__ __forceinline__ u16 f2bf_bits(float x) {
+  // round to nearest even (finite inputs)
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf_bits2f(u16 b) { return __uint_as_float(((unsigned)b) << 16); }
+// x ~ hi + lo with hi = bf16(x), lo = bf16(x - hi)
+__device__ __forceinline__ void split_bf16(double x, u16& hi, u16& lo) {
+  hi = f2bf_bits((float)x);
+  lo = f2bf_bits((float)(x - (double)bf_bits2f(hi)));
+}
+
+template <int BM, int BN, int WAVES_M, int WAVES_N, int NA, int NB>
+struct MmaBF16 {
+  static constexpr int BK = 32;
+  static constexpr int LDK = BK + 8;  // bf16 per LDS row (80 B, keeps 16-B alignment)
+  static constexpr int NT = 64 * WAVES_M * WAVES_N;
+  static constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  static constexpr int TM = WM / 16, TN = WN / 16;
+  static constexpr int A_CH = NA * BM * (BK / 8);  // 16-byte chunks per stage
+  static constexpr int B_CH = NB * BN * (BK / 8);
+  static constexpr int A_PER_T = (A_CH + NT - 1) / NT;
+  static constexpr int B_PER_T = (B_CH + NT - 1) / NT;
+  static constexpr int LDS_BYTES = (NA * BM + NB * BN) * LDK * 2;
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+
+  f32x4 acc[TM][TN];
+  int lane, wm, wn;
+
+  __device__ __forceinline__ MmaBF16() {
+    lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    wm = wave / WAVES_N;
+    wn = wave % WAVES_N;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+
+  // pa(c, r) / pb(c, col): const u16* to the K-contiguous row of component c, or nullptr.
+  // Kp: padded K (multiple of 8); rows hold at least Kp elements.
+  template <class PA, class PB>
+  __device__ __forceinline__ void run(const PA& pa, const PB& pb, int Kp, void* ldsv) {
+    u16* lds = reinterpret_cast<u16*>(ldsv);
+    u16* As = lds;                    // [NA][BM][LDK]
+    u16* Bs = lds + NA * BM * LDK;    // [NB][BN][LDK]
+    const int tid = threadIdx.x;
+    uint4 ra[A_PER_T], rb[B_PER_T];
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < A_PER_T; ++q) {
+        const int e = tid + q * NT;
+        ra[q] = zero;
+        if (e < A_CH) {
+          const int ch = e & 3, r = (e >> 2) % BM, c = (e >> 2) / BM;
+          const int k = k0 + ch * 8;
+          const u16* p = pa(c, r);
+          if (p && k < Kp) ra[q] = *reinterpret_cast<const uint4*>(p + k);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < B_PER_T; ++q) {
+        const int e = tid + q * NT;
+        rb[q] = zero;
+        if (e < B_CH) {
+          const int ch = e & 3, r = (e >> 2) % BN, c = (e >> 2) / BN;
+          const int k = k0 + ch * 8;
+          const u16* p = pb(c, r);
+          if (p && k < Kp) rb[q] = *reinterpret_cast<const uint4*>(p + k);
+        }
+      }
+    };
+    const int nst = (Kp + BK - 1) / BK;
+    if (nst > 0) fetch(0);
+    for (int s = 0; s < nst; ++s) {
+#pragma unroll
+      for (int q = 0; q < A_PER_T; ++q) {
+        const int e = tid + q * NT;
+        if (e < A_CH) {
+          const int ch = e & 3, r = (e >> 2) % BM, c = (e >> 2) / BM;
+          *reinterpret_cast<uint4*>(As + (c * BM + r) * LDK + ch * 8) = ra[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < B_PER_T; ++q) {
+        const int e = tid + q * NT;
+        if (e < B_CH) {
+          const int ch = e & 3, r = (e >> 2) % BN, c = (e >> 2) / BN;
+          *reinterpret_cast<uint4*>(Bs + (c * BN + r) * LDK + ch * 8) = rb[q];
+        }
+      }
+      __syncthreads();
+      if (s + 1 < nst) fetch((s + 1) * BK);
+      const int koff = 8 * (lane >> 4);
+      bf16x8 a[NA][TM], b[NB][TN];
+#pragma unroll
+      for (int c = 0; c < NA; ++c)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          a[c][i] = *reinterpret_cast<const bf16x8*>(As + (c * BM + wm * WM + i * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          b[c][j] = *reinterpret_cast<const bf16x8*>(Bs + (c * BN + wn * WN + j * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+          if (NB > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[NB - 1][j], acc[i][j], 0, 0, 0);
+          if (NA > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[NA - 1][i], b[0][j], acc[i][j], 0, 0, 0);
+        }
+      __syncthreads();
+    }
+  }
+
+  __device__ __forceinline__ int row(int i, int r) const { return wm * WM + i * 16 + 4 * (lane >> 4) + r; }
   __device__ __forceinline__ int col(int j) const { return wn * WN + j * 16 + (lane & 15); }
 
   template <class EP>

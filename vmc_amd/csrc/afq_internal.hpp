@@ -93,6 +93,8 @@ void launch_pack_chol(const double* L, double* Lp, int N, int G, int64_t T, bool
 void launch_rotate_chol(const double* L, const double* phi, double* R, int N, int n, int G, int64_t Ktot,
                         int64_t koff, hipStream_t st);
 void launch_debug_gemm(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t st);
+void launch_debug_gemm_bf16(const double* A, const double* B, float* C, int M, int N, int K, int passes,
+                            void* scratch, hipStream_t st);
 void launch_copy_calib(const double* a, double* b, int64_t n, hipStream_t st);
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
 void launch_zero_counter(unsigned long long* p, hipStream_t st);

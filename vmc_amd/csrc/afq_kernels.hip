@@ -43,10 +43,12 @@ struct FbB {  // B(k, g) = R[g][k]
 
 constexpr int FB_BM = 128, FB_BN = 128;
 __global__ __launch_bounds__(256, 2) void k_force_bias(const double* R, const double* th, double* fb,
-                                                    int G, int W, int64_t Ktot, double alpha) {
+                                                    int G, int W, int64_t Ktot, double alpha, int gridx,
+                                                    int gridy) {
   using Eng = MmaF64<FB_BM, FB_BN, 2, 2, false, false>;
   __shared__ double lds[Eng::LDS_DOUBLES];
-  const int row0 = blockIdx.y * FB_BM, g0 = blockIdx.x * FB_BN;
+  const TileId tl = tile_id(gridx, gridy, 1, 8);
+  const int row0 = tl.y * FB_BM, g0 = tl.x * FB_BN;
   Eng e;
   e.run(FbA{th, Ktot, row0, 2 * W}, FbB{R, Ktot, g0, G}, (int)Ktot, lds);
   e.store([&](int r, int c, double v) {
@@ -56,9 +58,9 @@ __global__ __launch_bounds__(256, 2) void k_force_bias(const double* R, const do
 }
 
 void launch_force_bias(const DevState& s, hipStream_t st) {
-  dim3 grid(cdivu(s.G, FB_BN), cdivu(2 * (int64_t)s.W, FB_BM));
-  hipLaunchKernelGGL(k_force_bias, grid, dim3(256), 0, st, s.R, s.theta, s.fb, s.G, s.W, s.Ktot,
-                     (double)s.slot[0].mult);
+  const int gx = (int)cdivu(s.G, FB_BN), gy = (int)cdivu(2 * (int64_t)s.W, FB_BM);
+  hipLaunchKernelGGL(k_force_bias, dim3(gx * gy), dim3(256), 0, st, s.R, s.theta, s.fb, s.G, s.W, s.Ktot,
+                     (double)s.slot[0].mult, gx, gy);
 }
 
 // ------------------------------------------------------------------ fields
@@ -147,11 +149,12 @@ struct VhsB {  // B(g, t) = Lp[g][t]
 constexpr int VHS_BM = 128, VHS_BN = 128;
 __global__ __launch_bounds__(256, 2) void k_vhs(const double* fields, const double* Lp, double* P,
                                              const double* weights, int G, int W, int64_t T, int N,
-                                             int ref_compat) {
+                                             int ref_compat, int gridx, int gridy) {
   using Eng = MmaF64<VHS_BM, VHS_BN, 2, 2, false, true>;
   __shared__ double lds[Eng::LDS_DOUBLES];
-  const int row0 = blockIdx.y * VHS_BM;
-  const int64_t t0 = (int64_t)blockIdx.x * VHS_BN;
+  const TileId tl = tile_id(gridx, gridy, 1, 8);
+  const int row0 = tl.y * VHS_BM;
+  const int64_t t0 = (int64_t)tl.x * VHS_BN;
   // skip tiles whose walkers are all dead
   bool any = false;
   for (int q = row0; q < min(row0 + VHS_BM, 2 * W); q += 2) any |= weights[q >> 1] > 1e-8;
@@ -175,9 +178,9 @@ __global__ __launch_bounds__(256, 2) void k_vhs(const double* fields, const doub
 }
 
 void launch_vhs(const DevState& s, hipStream_t st) {
-  dim3 grid(cdivu(s.T, VHS_BN), cdivu(2 * (int64_t)s.W, VHS_BM));
-  hipLaunchKernelGGL(k_vhs, grid, dim3(256), 0, st, s.fields, s.Lp, s.P, s.weights, s.G, s.W, s.T, s.N,
-                     (int)s.ref_compat);
+  const int gx = (int)cdivu(s.T, VHS_BN), gy = (int)cdivu(2 * (int64_t)s.W, VHS_BM);
+  hipLaunchKernelGGL(k_vhs, dim3(gx * gy), dim3(256), 0, st, s.fields, s.Lp, s.P, s.weights, s.G, s.W, s.T,
+                     s.N, (int)s.ref_compat, gx, gy);
 }
 
 // ------------------------------------------------------------------ propagation
@@ -202,10 +205,12 @@ constexpr int EX_BM = 64, EX_BN = 128;
 // mode 0: dst1 = dst2 = E src (start of the Taylor series); mode 1: dst1 = E src for live walkers
 __global__ __launch_bounds__(256) void k_expmul(const double* E, const double* src, double* dst1,
                                                 double* dst2, const double* weights, int N, int n,
-                                                int W, int64_t wstride, int64_t doff, int mode) {
+                                                int W, int64_t wstride, int64_t doff, int mode, int gridx,
+                                                int gridy) {
   using Eng = MmaF64<EX_BM, EX_BN, 2, 2, false, true>;
   __shared__ double lds[Eng::LDS_DOUBLES];
-  const int row0 = blockIdx.y * EX_BM, q0 = blockIdx.x * EX_BN;
+  const TileId tl = tile_id(gridx, gridy, 1, gridy);
+  const int row0 = tl.y * EX_BM, q0 = tl.x * EX_BN;
   const int n2 = 2 * n, ncol = W * n2;
   Eng e;
   e.run(ExpA{E, N, row0}, DetColsB{src, wstride, doff, N, n2, q0, ncol}, N, lds);
@@ -248,12 +253,13 @@ constexpr int VM_BM = 64, VM_BN = 128;
 __global__ __launch_bounds__(256) void k_vmul(const double* P, const double* tsrc, double* tdst,
                                               double* acc, const double* weights, int N, int64_t T,
                                               int n2a, int n2b, int64_t offb, int64_t wstride,
-                                              double sq, double inv_p) {
+                                              double sq, double inv_p, int gridx, int gridy, int W) {
   using Eng = MmaF64<VM_BM, VM_BN, 2, 2, false, true>;
   __shared__ double lds[Eng::LDS_DOUBLES];
-  const int w = blockIdx.z;
+  const TileId tl = tile_id(gridx, gridy, W, gridy);
+  const int w = tl.z;
   if (weights[w] <= 1e-8) return;
-  const int row0 = blockIdx.y * VM_BM, q0 = blockIdx.x * VM_BN;
+  const int row0 = tl.y * VM_BM, q0 = tl.x * VM_BN;
   const int ncol = n2a + n2b;
   const double* Pre = P + (int64_t)(2 * w) * N * N;
   const double* Pim = Pre + (int64_t)N * N;
@@ -277,9 +283,9 @@ void launch_propagate(const DevState& s, hipStream_t st) {
   const int64_t ws = 2 * s.ms;
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
-    dim3 grid(cdivu((int64_t)s.W * 2 * n, EX_BN), cdivu(s.N, EX_BM));
-    hipLaunchKernelGGL(k_expmul, grid, dim3(256), 0, st, s.E, s.psi, s.t0, s.accb, s.weights, s.N, n,
-                       s.W, ws, 2 * s.det_off[d], 0);
+    const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
+    hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.psi, s.t0, s.accb, s.weights, s.N,
+                       n, s.W, ws, 2 * s.det_off[d], 0, gx, gy);
   }
   const int n2a = 2 * s.det_n[0];
   const int n2b = s.ndets_prop > 1 ? 2 * s.det_n[1] : 0;
@@ -288,16 +294,16 @@ void launch_propagate(const DevState& s, hipStream_t st) {
   double* src = s.t0;
   double* dst = s.t1;
   for (int p = 1; p <= 5; ++p) {
-    dim3 grid(cdivu(n2a + n2b, VM_BN), cdivu(s.N, VM_BM), s.W);
-    hipLaunchKernelGGL(k_vmul, grid, dim3(256), 0, st, s.P, src, dst, s.accb, s.weights, s.N, s.T, n2a,
-                       n2b, offb, ws, sq, 1.0 / p);
+    const int gx = (int)cdivu(n2a + n2b, VM_BN), gy = (int)cdivu(s.N, VM_BM);
+    hipLaunchKernelGGL(k_vmul, dim3(gx * gy * s.W), dim3(256), 0, st, s.P, src, dst, s.accb, s.weights, s.N,
+                       s.T, n2a, n2b, offb, ws, sq, 1.0 / p, gx, gy, s.W);
     double* t = src; src = dst; dst = t;
   }
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
-    dim3 grid(cdivu((int64_t)s.W * 2 * n, EX_BN), cdivu(s.N, EX_BM));
-    hipLaunchKernelGGL(k_expmul, grid, dim3(256), 0, st, s.E, s.accb, s.psi, nullptr, s.weights, s.N,
-                       n, s.W, ws, 2 * s.det_off[d], 1);
+    const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
+    hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.accb, s.psi, nullptr, s.weights,
+                       s.N, n, s.W, ws, 2 * s.det_off[d], 1, gx, gy);
   }
 }
 
@@ -648,14 +654,16 @@ struct XcB {  // B(k, (w,j,c)) = theta_s[w][k][j][c]
 // and keeps a running total, so f never leaves the workgroup.
 __global__ __launch_bounds__(256, 2) void k_exchange(const double* R, const double* theta, const int* sel,
                                                      double* Xp, int N, int GE, int nw, int W, int64_t Ktot,
-                                                     Slot s0, Slot s1, int ngt) {
+                                                     Slot s0, Slot s1, int ngt, int gridx, int gridy,
+                                                     int gridz) {
   using Eng = MmaF64<XC_BM, XC_BN, 2, 2, true, true>;
   constexpr int FT = (XC_BN / 2) * XC_BN;
   constexpr int LDSN = Eng::LDS_DOUBLES > FT ? Eng::LDS_DOUBLES : FT;
   __shared__ double lds[LDSN];
-  const Slot sl = blockIdx.z == 0 ? s0 : s1;
+  const TileId tl = tile_id(gridx, gridy, gridz, 8);
+  const Slot sl = tl.z == 0 ? s0 : s1;
   const int n = sl.n, gt = XC_BM / n, wt = XC_BN / (2 * n);
-  const int g0 = blockIdx.x * gt, w0 = blockIdx.y * wt;
+  const int g0 = tl.x * gt, w0 = tl.y * wt;
   Eng e;
   e.run(XcA{R, Ktot, sl.koff, n, gt, g0, GE, N}, XcB{theta, sel, Ktot, sl.koff, n, wt, w0, nw, N}, N, lds);
   const int tid = threadIdx.x, nr = wt * n;
@@ -694,7 +702,7 @@ __global__ __launch_bounds__(256, 2) void k_exchange(const double* R, const doub
       si += lds[2 * (tid * n + i) + 1];
     }
     const int w = sel ? sel[w0 + tid] : w0 + tid;
-    double* o = Xp + 2 * (((int64_t)blockIdx.z * W + w) * ngt + blockIdx.x);
+    double* o = Xp + 2 * (((int64_t)tl.z * W + w) * ngt + tl.x);
     o[0] = sr;
     o[1] = si;
   }
@@ -705,19 +713,20 @@ void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st
   if (s.nslots == 1 || s.slot[1].n == n) {
     const int gt = exchange_tile_g(n), wt = exchange_tile_w(n);
     const int ngt = (int)cdivu(s.GE, gt);
-    dim3 grid(ngt, cdivu(nsel, wt), s.nslots);
-    hipLaunchKernelGGL(k_exchange, grid, dim3(256), 0, st, s.R, s.theta, sel, s.Xp, s.N, s.GE, nsel, s.W,
-                       s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ngt_cap);
+    const int gy = (int)cdivu(nsel, wt);
+    hipLaunchKernelGGL(k_exchange, dim3(ngt * gy * s.nslots), dim3(256), 0, st, s.R, s.theta, sel, s.Xp, s.N,
+                       s.GE, nsel, s.W, s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ngt_cap, ngt, gy,
+                       s.nslots);
   } else {
     for (int z = 0; z < s.nslots; ++z) {
       const int nz = s.slot[z].n;
       const int gt = exchange_tile_g(nz), wt = exchange_tile_w(nz);
       const int ngt = (int)cdivu(s.GE, gt);
       // a one-slot grid whose z index is the slot: pass the slot as s0 and offset Xp
-      dim3 grid(ngt, cdivu(nsel, wt), 1);
-      hipLaunchKernelGGL(k_exchange, grid, dim3(256), 0, st, s.R, s.theta, sel,
+      const int gy = (int)cdivu(nsel, wt);
+      hipLaunchKernelGGL(k_exchange, dim3(ngt * gy), dim3(256), 0, st, s.R, s.theta, sel,
                          s.Xp + 2 * (int64_t)z * s.W * s.ngt_cap, s.N, s.GE, nsel, s.W, s.Ktot, s.slot[z],
-                         s.slot[z], s.ngt_cap);
+                         s.slot[z], s.ngt_cap, ngt, gy, 1);
     }
   }
 }
@@ -899,6 +908,50 @@ __global__ __launch_bounds__(256) void k_debug_gemm(const double* A, const doubl
 }
 void launch_debug_gemm(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t st) {
   hipLaunchKernelGGL(k_debug_gemm, dim3(cdivu(N, 128), cdivu(M, 64)), dim3(256), 0, st, A, B, C, M, N, K);
+}
+
+// split an FP64 matrix M x K (row-major) into K-contiguous bf16 hi/lo rows with padded K
+__global__ void k_split_rows(const double* X, u16* hi, u16* lo, int M, int K, int Kp, int transpose) {
+  // transpose == 0: row m of X -> row m;  transpose == 1: X is K x M (row-major), column m -> row m
+  const int64_t total = (int64_t)M * Kp;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(e / Kp), k = (int)(e % Kp);
+    double x = 0.0;
+    if (k < K) x = transpose ? X[(int64_t)k * M + m] : X[(int64_t)m * K + k];
+    u16 h, l;
+    split_bf16(x, h, l);
+    hi[e] = h;
+    if (lo) lo[e] = l;
+  }
+}
+template <int NS>
+__global__ __launch_bounds__(256) void k_debug_gemm_bf16(const u16* Ah, const u16* Al, const u16* Bh, const u16* Bl,
+                                                          float* C, int M, int N, int Kp) {
+  using Eng = MmaBF16<64, 128, 2, 2, NS, NS>;
+  __shared__ __attribute__((aligned(16))) u16 lds[Eng::LDS_BYTES / 2];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 128;
+  Eng e;
+  e.run([&](int c, int r) -> const u16* { return r0 + r < M ? (c ? Al : Ah) + (int64_t)(r0 + r) * Kp : nullptr; },
+        [&](int c, int q) -> const u16* { return c0 + q < N ? (c ? Bl : Bh) + (int64_t)(c0 + q) * Kp : nullptr; },
+        Kp, lds);
+  e.store([&](int r, int c, float v) {
+    if (r0 + r < M && c0 + c < N) C[(int64_t)(r0 + r) * N + c0 + c] = v;
+  });
+}
+void launch_debug_gemm_bf16(const double* A, const double* B, float* C, int M, int N, int K, int passes,
+                            void* scratch, hipStream_t st) {
+  const int Kp = (K + 7) / 8 * 8;
+  u16* Ah = (u16*)scratch;
+  u16* Al = Ah + (int64_t)M * Kp;
+  u16* Bh = Al + (int64_t)M * Kp;
+  u16* Bl = Bh + (int64_t)N * Kp;
+  hipLaunchKernelGGL(k_split_rows, dim3(256), dim3(256), 0, st, A, Ah, Al, M, K, Kp, 0);
+  hipLaunchKernelGGL(k_split_rows, dim3(256), dim3(256), 0, st, B, Bh, Bl, N, K, Kp, 1);
+  dim3 grid(cdivu(N, 128), cdivu(M, 64));
+  if (passes == 3)
+    hipLaunchKernelGGL(k_debug_gemm_bf16<2>, grid, dim3(256), 0, st, Ah, Al, Bh, Bl, C, M, N, Kp);
+  else
+    hipLaunchKernelGGL(k_debug_gemm_bf16<1>, grid, dim3(256), 0, st, Ah, Al, Bh, Bl, C, M, N, Kp);
 }
 
 // plain streaming copy (read n, write n doubles): the HBM-traffic calibration kernel
