@@ -57,19 +57,56 @@ def flops_per_walker_step(spec, ortho=5, energy=True):
     return f
 
 
-def run_path(ham, spec, steps, warmup, **prec):
+STAGES = ["force_bias", "fields", "vhs", "onebody", "green", "weight", "ortho", "exchange", "comb", "taylor"]
+PEAKS = {"fp64": 78.6, "bf16": 2500.0}
+
+
+def run_path(ham, spec, total_steps, warmup, **kw):
+    """Driver run with E_L every step (nsteps=1), device RNG; the first `warmup`
+    steps are excluded from timing."""
     from vmc_amd.afqmc import AFQ_RNG_DEVICE, run_long_prop
     W = spec["nwalk"]
     w, e, res = run_long_prop(
         ham, ham.phi_up, ham.phi_dn, ham.phi_up, ham.phi_dn, seed=20261202, dt=spec["dt"], nsteps=1,
-        nsweeps=warmup + steps + 1, ortho_steps=spec["ortho"], nwalk=W, nranks=1,
+        nsweeps=total_steps + 1, ortho_steps=spec["ortho"], nwalk=W, nranks=1,
         walker_rhf=spec["walker_rhf"], cholesky_threshold=0.0, rng_mode=AFQ_RNG_DEVICE,
-        warmup_steps=warmup, **prec)
+        warmup_steps=warmup, **kw)
     k = int(res.timed_steps)
     wall = float(res.timed_wall_s)
-    return dict(walker_steps_per_s=W * k / wall, ms_per_step=1e3 * wall / k, steps=k,
-                energies=e[warmup + 1:], weights=w[warmup + 1:], stage_ms=list(res.stage_ms),
-                nchol_ene=int(res.nchol_ene))
+    ms = 1e3 * wall / k
+    return dict(walker_steps_per_s=W * k / wall, ms_per_step=ms, steps=k, energies=np.array(e),
+                weights=np.array(w), stage_ms_per_step={n: v / k for n, v in zip(STAGES, res.stage_ms)},
+                est_var=float(res.est_var), err_per_sqrt_s=float(np.sqrt(res.est_var * ms * 1e-3)),
+                train_rmse=float(res.train_rmse), valid_rmse=float(res.valid_rmse),
+                cv_var_sub=float(res.cv_var_sub), cv_rms=float(res.cv_rms),
+                cv_fallbacks=int(res.cv_fallbacks), train_samples=int(res.train_samples))
+
+
+def stage_rooflines(spec, path, n_exact):
+    """Algorithmic FLOP/s of each stage of one step (per launch for the kernels that
+    launch several times) against the MFMA peak of the precision it runs in."""
+    N, G, W = spec["norb"], spec["nchol"], spec["nwalk"]
+    T = N * (N + 1) / 2
+    nsp = [spec["nalpha"]] if spec["walker_rhf"] else [spec["nalpha"], spec["nbeta"]]
+    st = path["stage_ms_per_step"]
+    out = {}
+    def add(name, kernel, flops, launches, prec):
+        ms = st[name] / launches
+        if ms <= 0:
+            return
+        ach = flops / launches / (ms * 1e-3) / 1e12
+        out[name] = dict(kernel=kernel, launch_ms=ms, flops_per_launch=flops / launches,
+                         achieved=ach, peak=PEAKS[prec], frac=ach / PEAKS[prec], unit="TFLOP/s",
+                         bound="mfma", precision=prec)
+    add("exchange", "k_exchange / k_exchange_bf16", exchange_flops(spec, n_exact), 1,
+        "fp64" if path["exchange_prec"] == "fp64" else "bf16")
+    add("taylor", "k_vmul (5 complex V t products, FP64 MFMA)", W * sum(40.0 * N * N * n for n in nsp), 5, "fp64")
+    add("onebody", "k_expmul (E psi, FP64 MFMA)", W * sum(8.0 * N * N * n for n in nsp), 2 * len(nsp), "fp64")
+    add("force_bias", "k_force_bias (FP64 MFMA)", W * sum(4.0 * n * N * G for n in nsp), 1, "fp64")
+    add("vhs", "k_vhs (FP64 MFMA)", W * 4.0 * T * G, 1, "fp64")
+    add("green", "k_green (FP64 MFMA + Gauss-Jordan)", W * sum(4 * n * n * N + 16 * n ** 3 + 8 * N * n * n
+                                                              for n in nsp), 1, "fp64")
+    return out
 
 
 def cpu_baseline(ham, spec, threads):
@@ -93,6 +130,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c4")
+    ap.add_argument("--train-steps", type=int, default=0,
+                    help="exact FP64 steps whose (walker, E_K - E_J) pairs train the surrogate "
+                         "(default: ~48k pairs)")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--subset-inv", type=int, default=16)
+    ap.add_argument("--risk-frac", type=float, default=0.05,
+                    help="fraction of live walkers (largest |E_J - median|) always measured exactly")
+    ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
@@ -100,28 +145,62 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs one GPU holding one rank's walker population; "
-              "--gpus %d is not supported (multi-GPU sharding is described in DESIGN.md)" % a.gpus,
+              "--gpus %d is not supported" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    from vmc_amd.abi import AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16X3
     from vmc_amd.synthetic import workload
     t0 = time.time()
     ham, spec = workload(a.workload)
-    spec["nwalk"] = spec["nwalk"]
+    W = spec["nwalk"]
     print("[bench] workload %s built in %.1fs" % (a.workload, time.time() - t0), file=sys.stderr, flush=True)
     if a.calibrate:
         from vmc_amd.afqmc import debug_copy
         debug_copy(1 << 30)
-    fp64 = run_path(ham, spec, a.steps, a.warmup)
+    T = a.train_steps or max(2, (48 * 1024 + W - 1) // W)
+    if a.fp64_only:
+        T = 0
+    total = T + a.warmup + a.steps
+    fp64 = run_path(ham, spec, total, T + a.warmup)
+    fp64["exchange_prec"] = "fp64"
     print("[bench] fp64: %.1f walker-steps/s, %.1f ms/step" % (fp64["walker_steps_per_s"], fp64["ms_per_step"]),
           file=sys.stderr, flush=True)
+    paths = {"fp64": fp64}
     best = fp64
-    W = spec["nwalk"]
-    xms = fp64["stage_ms"][7] / fp64["steps"]  # one exchange launch per step
-    xflops = exchange_flops(spec, W)
-    achieved = xflops / (xms * 1e-3) / 1e12
-    roof = dict(bound="mfma", kernel="k_exchange (FP64 MFMA 16x16x4)", achieved=achieved,
-                peak=PEAK_FP64_TFLOPS, unit="TFLOP/s", frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
-                launch_ms=xms, flops_per_launch=xflops)
+    if not a.fp64_only:
+        mix = run_path(ham, spec, total, T + a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
+                       prec_exchange=AFQ_PREC_BF16X3, surrogate_train_steps=T, surrogate_hidden=256,
+                       surrogate_epochs=a.epochs, subset_inv=a.subset_inv, risk_frac=a.risk_frac)
+        mix["exchange_prec"] = "bf16x3"
+        print("[bench] mixed+surrogate: %.1f walker-steps/s, %.1f ms/step (train rmse %.3g, valid %.3g Ha)"
+              % (mix["walker_steps_per_s"], mix["ms_per_step"], mix["train_rmse"], mix["valid_rmse"]),
+              file=sys.stderr, flush=True)
+        paths["mixed_surrogate"] = mix
+        best = mix
+    for name, pth in paths.items():
+        nrisk = int(np.ceil(a.risk_frac * W))
+        n_exact = W if name == "fp64" else nrisk + (W - nrisk + a.subset_inv - 1) // a.subset_inv
+        pth["rooflines"] = stage_rooflines(spec, pth, n_exact)
+    # the dominant kernel: largest share of the step
+    shares = {k: best["stage_ms_per_step"][k] for k in best["rooflines"]}
+    dom_name = max(shares, key=shares.get)
+    dom = dict(best["rooflines"][dom_name])
+    dom["stage"] = dom_name
+    dom["traffic"] = None
+    result_energy = {}
+    if "mixed_surrogate" in paths:
+        e64, emx = fp64["energies"], paths["mixed_surrogate"]["energies"]
+        d = (emx - e64)[T + 1:]
+        result_energy = dict(
+            energy_shift_mha=1e3 * float(np.mean(d)), energy_shift_sem_mha=1e3 * float(np.std(d, ddof=1) / np.sqrt(len(d)))
+            if len(d) > 1 else None, max_abs_step_shift_mha=1e3 * float(np.max(np.abs(d))), compared_steps=len(d),
+            speedup_vs_fp64=paths["mixed_surrogate"]["walker_steps_per_s"] / fp64["walker_steps_per_s"],
+            err_per_sqrt_s_ratio=paths["mixed_surrogate"]["err_per_sqrt_s"] / fp64["err_per_sqrt_s"]
+            if fp64["err_per_sqrt_s"] > 0 else None)
+        print("[bench] speedup %.2fx, energy shift %.4f mHa (sem %s), error/sqrt(s) ratio %s"
+              % (result_energy["speedup_vs_fp64"], result_energy["energy_shift_mha"],
+                 result_energy["energy_shift_sem_mha"], result_energy["err_per_sqrt_s_ratio"]),
+              file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
@@ -129,33 +208,34 @@ def main():
         print("[bench] cpu baseline: %.2f walker-steps/s on %d threads" % (cpu["value"], threads),
               file=sys.stderr, flush=True)
     wstep = flops_per_walker_step(spec)
+    for pth in paths.values():
+        pth.pop("energies", None)
+        pth.pop("weights", None)
+        pth["step_tflops_algorithmic"] = wstep * pth["walker_steps_per_s"] / 1e12
     out = {
         "metric": METRIC,
         "value": best["walker_steps_per_s"],
         "unit": "walker-steps/s",
         "n_gpus": 1,
         "steps": best["steps"],
-        "warmup": a.warmup,
+        "warmup": T + a.warmup,
         "ms_per_step": best["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+bf16x3" if "mixed_surrogate" in paths else "f64",
         "data": "synthetic",
         "config": {"workload": "N=%d %da+%db G=%d %s trial, %d walkers, dt=%g, E_L every step, QR every %d"
                    % (spec["norb"], spec["nalpha"], spec["nbeta"], spec["nchol"], spec["trial"].upper(),
                       W, spec["dt"], spec["ortho"]),
-                   "name": a.workload, "norb": spec["norb"], "nalpha": spec["nalpha"],
-                   "nbeta": spec["nbeta"], "nchol": spec["nchol"], "nwalk": W},
-        "roofline": roof,
+                   "name": a.workload, "path": "mixed_surrogate" if "mixed_surrogate" in paths else "fp64",
+                   "subset_inv": a.subset_inv, "risk_frac": a.risk_frac, "surrogate_train_steps": T},
+        "roofline": dom,
         "cpu_baseline": cpu,
-        "paths": {"fp64": {"walker_steps_per_s": fp64["walker_steps_per_s"], "ms_per_step": fp64["ms_per_step"],
-                           "stage_ms_per_step": dict(zip(
-                               ["force_bias", "fields", "vhs", "propagate", "green", "weight", "ortho",
-                                "exchange", "comb"], [x / fp64["steps"] for x in fp64["stage_ms"]])),
-                           "step_tflops": wstep * fp64["walker_steps_per_s"] / 1e12}},
+        "paths": paths,
         "flops_per_walker_step": wstep,
     }
+    out.update(result_energy)
     print(json.dumps(out))
 
 

@@ -122,6 +122,11 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
                afq_ctx** out);
 int afq_destroy(afq_ctx* ctx);
 
+/* Change the arithmetic of the stages on an existing context (the walker
+ * population is kept).  This build supports prec_fb = prec_vhs = AFQ_PREC_FP64
+ * and prec_exchange in {FP64, BF16X3, BF16}. */
+int afq_set_precision(afq_ctx* ctx, int32_t prec_fb, int32_t prec_vhs, int32_t prec_exchange);
+
 /* Number of Cholesky vectors used by the local energy (default nchol). */
 int afq_set_nchol_ene(afq_ctx* ctx, int32_t nchol_ene);
 
@@ -178,9 +183,48 @@ int afq_overlap(afq_ctx* ctx, double* overlaps);
  * weight to sum/W.  total_weight (nullable) receives sum. */
 int afq_comb(afq_ctx* ctx, double zeta, double* total_weight);
 
-/* Per-stage device time (ms) accumulated since the last reset, in the order
- * force_bias, fields, vhs, propagate, green, weight, ortho, energy, comb. */
-int afq_stage_times(afq_ctx* ctx, double* ms9, int32_t reset);
+/* Per-stage device time (ms) accumulated since the last reset, AFQ_NSTAGES
+ * entries in the order force_bias, fields, vhs, one-body (E psi), green, weight,
+ * ortho, exchange, comb, taylor (the five V t products). */
+#define AFQ_NSTAGES 10
+int afq_stage_times(afq_ctx* ctx, double* ms10, int32_t reset);
+
+/* ---------------------------------------------------------------- surrogate
+ * Exchange-energy surrogate (no counterpart in the reference): a GELU MLP
+ * F -> hidden -> hidden -> hidden -> 1, F = nchol + 2 norb, fed Re fb_g and
+ * Re diag G_up/dn, trained on the GPU (BF16 MFMA GEMMs, FP32 accumulation,
+ * FP32 master weights, Adam, MSE) on (walker features, exact E_K - E_J) pairs
+ * that afq_surrogate_collect takes from exact measurements.  The risk stratum,
+ * the fraction risk_frac of live walkers whose Coulomb energy E_J lies farthest
+ * from the median, is always evaluated exactly (and is left out of training).
+ * afq_measure with AFQ_ENERGY_SURROGATE then evaluates E_L = E_1 - S(x) on every
+ * walker plus pinv (S - (E_K - E_J)) on the exact set: the risk stratum
+ * (pinv = 1) and one walker drawn from each block of subset_inv remaining live
+ * walkers (pinv = block size), an unbiased Horvitz-Thompson estimate of every
+ * walker's exact E_L.  Every walker is measured exactly instead when the random
+ * part's residual RMS exceeds fallback_rms (0: never). */
+int afq_surrogate_init(afq_ctx* ctx, int32_t hidden, uint64_t seed, int64_t capacity);
+int afq_surrogate_collect(afq_ctx* ctx, int64_t* total);
+/* append n external (features X[n x F], target Y[n]) pairs to the training set */
+int afq_surrogate_load(afq_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t* total);
+int afq_surrogate_train(afq_ctx* ctx, int32_t epochs, int32_t batch, double lr, uint64_t seed,
+                        double* train_rmse, double* valid_rmse);
+int afq_surrogate_predict(afq_ctx* ctx, double* pred);
+/* flattened parameters: W0 b0 W1 b1 W2 b2 W3 b3 (W_l row-major in x out), mu[F], sd[F], t_mu, t_sd */
+int afq_surrogate_params(afq_ctx* ctx, float* buf, int64_t nbuf, int32_t set);
+/* one forward + backward pass on raw features X (B x F) and targets t (B): gradients of
+ * 1/2 mean((y - (t - t_mu)/t_sd)^2) in the order of afq_surrogate_params (weights/biases only) */
+int afq_surrogate_grad(afq_ctx* ctx, const float* X, const float* t, int32_t B, float* grads, double* loss);
+int afq_set_cv(afq_ctx* ctx, int32_t subset_inv, double risk_frac, double fallback_rms, uint64_t seed);
+/* per-walker one-body energy sum_s m_s tr(G_s h1) and exchange energy E_K (real
+ * parts) of the last exact measurement of every walker */
+int afq_energy_parts(afq_ctx* ctx, double* e1, double* ek);
+/* surrogate input features of every walker (nwalk x (nchol + 2 norb), float) */
+int afq_surrogate_features(afq_ctx* ctx, float* X);
+/* {exact-set size, residual RMS of its random part (Ha), fallback used, subset-sampling
+ *  variance estimate of the weighted mean energy (Ha^2), mean residual, cumulative
+ *  fallbacks} of the last surrogate measurement */
+int afq_cv_stats(afq_ctx* ctx, double* out6);
 
 /* ---------------------------------------------------------------- driver
  * calcMixedEstimatorLongProp (DQMC/MixedEstimator.cpp:309-699) on one GPU.
@@ -212,6 +256,15 @@ typedef struct afq_run_params {
   int32_t prec_fb, prec_vhs, prec_exchange;
   int32_t energy_mode;   /* AFQ_ENERGY_*                                      */
   int32_t device;
+  /* surrogate control variate (energy_mode == AFQ_ENERGY_SURROGATE): the first
+   * surrogate_train_steps steps measure exactly (FP64) and collect (features,
+   * E_K) pairs, then the MLP is trained and the control variate takes over */
+  int32_t surrogate_train_steps;
+  int32_t surrogate_hidden;
+  int32_t surrogate_epochs;
+  int32_t subset_inv;    /* exact E_L on a 1/subset_inv stratified subset     */
+  double fallback_rms;   /* exact fallback when the subset residual RMS exceeds this (0: never) */
+  double risk_frac;      /* fraction of live walkers always measured exactly  */
 } afq_run_params;
 
 typedef struct afq_run_result {
@@ -222,7 +275,13 @@ typedef struct afq_run_result {
   double prop_time, ene_time;  /* host wall seconds, as DQMC/MixedEstimator.cpp:664-676 */
   int64_t timed_steps;
   double timed_wall_s;         /* wall seconds of the steps after warmup_steps     */
-  double stage_ms[9];          /* device time per stage (afq_stage_times order)     */
+  double stage_ms[10];         /* device time per stage (afq_stage_times order)     */
+  double train_rmse, valid_rmse; /* surrogate fit (Ha)                             */
+  double cv_var_sub;           /* mean subset-sampling variance of the block energy */
+  double cv_rms;               /* mean residual RMS of the exact subset (Ha)        */
+  int64_t cv_fallbacks;
+  int64_t train_samples;
+  double est_var;              /* mean over timed blocks of sum w^2 (E_L - E)^2 / (sum w)^2 (Ha^2) */
 } afq_run_result;
 
 /* ref_up/ref_dn: reference determinant (rhf.txt or uhf.txt by walker type,

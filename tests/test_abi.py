@@ -1,0 +1,60 @@
+"""The C-ABI library (no GPU needed): it loads, exports every function that
+include/vmc_afqmc.h declares, and the ctypes mirrors of the header's structs
+have the C layout (sizes and field offsets measured by a C program compiled
+from the header with gcc)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vmc_afqmc.h")
+LIB = os.path.join(ROOT, "vmc_amd", "libvmc_afqmc.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(afq_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_functions():
+    names = declared_functions()
+    assert "afq_create" in names and "afq_propagate" in names and "afq_run_long_prop" in names
+    assert len(names) >= 35
+
+
+def test_library_exports_every_declared_function():
+    if not os.path.exists(LIB):
+        pytest.fail("vmc_amd/libvmc_afqmc.so is not built (run __graft_entry__.build())")
+    lib = C.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.afq_version() > 0
+
+
+def c_layout(tmp_path, struct, fields):
+    src = tmp_path / "layout.c"
+    body = "\n".join('printf("%%zu\\n", offsetof(%s, %s));' % (struct, f) for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "vmc_afqmc.h"\n'
+                   'int main(void){printf("%%zu\\n", sizeof(%s));%s return 0;}\n' % (struct, body))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    return out[0], out[1:]
+
+
+@pytest.mark.parametrize("cname,pyname", [("afq_ham", "AfqHam"), ("afq_trial", "AfqTrial"),
+                                          ("afq_params", "AfqParams"), ("afq_run_params", "RunParams"),
+                                          ("afq_run_result", "RunResult")])
+def test_struct_layout_matches_header(tmp_path, cname, pyname):
+    from vmc_amd import abi, afqmc
+    py = getattr(abi, pyname, None) or getattr(afqmc, pyname)
+    fields = [f[0] for f in py._fields_]
+    size, offs = c_layout(tmp_path, cname, fields)
+    assert C.sizeof(py) == size
+    assert [getattr(py, f).offset for f in fields] == offs

@@ -1,0 +1,189 @@
+"""Learned exchange-energy surrogate and the two-level control variate.
+
+* gradients of the GPU forward/backward kernels against the numpy oracle
+  (oracle/mlp_oracle.py, same bf16 operand rounding, float64 accumulation);
+* the GPU training loop fits a learnable target (a fixed teacher function of
+  the inputs) to a small fraction of its spread;
+* on walker data (target E_K - E_J of the tame walkers) the kept network is no
+  worse than the constant predictor on held-out walkers;
+* the control-variate energy is unbiased: averaged over subset draws it equals
+  the exact population energy, also with an untrained (bad) surrogate; with
+  every walker in the exact set (risk fraction 1, or blocks of one) it equals
+  the exact energies walker by walker, and the fallback returns them too.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import build, perturbed_walkers
+from vmc_amd.abi import AFQ_ENERGY_SURROGATE
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def e_ref(ctx, h):
+    o = ctx.ham_and_overlap(h.serial_det(h.phi_up, h.phi_dn)[None, :], h.nchol)[0]
+    return (o[0] / o[1]).real
+
+
+def grow_population(ctx, h, steps, rng):
+    """phaseless steps with eshift at the trial energy, QR every 5 steps and a comb
+    every step (DQMC/MixedEstimator.cpp:455-635): the population stays physical"""
+    es = e_ref(ctx, h)
+    for s in range(steps):
+        ctx.propagate(rng.standard_normal((ctx.nwalk, h.nchol)), es)
+        if (s + 1) % 5 == 0:
+            ctx.orthogonalize()
+        ctx.comb(float(rng.uniform()))
+
+
+def test_surrogate_gradients_vs_oracle():
+    h, wrhf, ctx, orc = build("c1_small", 8)
+    ctx.set_walkers(perturbed_walkers(h, 8, wrhf, np.random.default_rng(0), 0.05))
+    ctx.surrogate_init(hidden=64, seed=3)
+    F = h.nchol + 2 * h.norb
+    dims = [F, 64, 64, 64, 1]
+    rng = np.random.default_rng(1)
+    p = ctx.surrogate_params()
+    # a non-trivial normalisation and target scaling
+    nw = sum(dims[l] * dims[l + 1] + dims[l + 1] for l in range(4))
+    p[nw:nw + F] = rng.normal(0, 0.1, F)
+    p[nw + F:nw + 2 * F] = rng.uniform(0.5, 2.0, F)
+    p[-2], p[-1] = 3.0, 2.0
+    ctx.surrogate_params(p)
+    X = rng.standard_normal((50, F)).astype(np.float32)
+    t = rng.normal(3.0, 2.0, 50).astype(np.float32)
+    loss_g, g = ctx.surrogate_grad(X, t)
+    loss_o, go = mlp_oracle.loss_and_grads(p, dims, X, t)
+    assert abs(loss_g - loss_o) < 1e-3 * abs(loss_o)
+    off = 0
+    for l in range(4):
+        for n in (dims[l] * dims[l + 1], dims[l + 1]):
+            a, b = g[off:off + n], go[off:off + n]
+            assert np.linalg.norm(a - b) < 2e-2 * np.linalg.norm(b) + 1e-7, (l, n, np.linalg.norm(a - b) / np.linalg.norm(b))
+            off += n
+
+
+def exact_y(ctx, h):
+    """E_K - E_J of every walker from the last exact measurement"""
+    _, ek = ctx.energy_parts()
+    fb = ctx.force_bias()
+    ej = 0.5 * np.sum((fb[:, :h.nchol] ** 2).real, axis=1)
+    return ek - ej
+
+
+def test_training_fits_teacher():
+    h, wrhf, ctx, orc = build("c1_small", 64)
+    F = h.nchol + 2 * h.norb
+    rng = np.random.default_rng(11)
+    n = 8192
+    X = rng.standard_normal((n, F)).astype(np.float32) * 3.0 + 1.0
+    v1, v2 = rng.standard_normal(F), rng.standard_normal(F)
+    v1 /= np.linalg.norm(v1)
+    v2 /= np.linalg.norm(v2)
+    z1, z2 = (X - 1.0) / 3.0 @ v1, (X - 1.0) / 3.0 @ v2
+    Y = (50.0 + 20.0 * z1 + 5.0 * z2 ** 2).astype(np.float32)
+    ctx.surrogate_init(hidden=128, seed=12, capacity=n)
+    assert ctx.surrogate_load(X, Y) == n
+    tr, va = ctx.surrogate_train(epochs=30, batch=256, lr=1e-3, seed=13)
+    print("teacher: sd %.3f  train rmse %.3f  valid rmse %.3f" % (Y.std(), tr, va))
+    assert va < 0.3 * Y.std() and tr < 0.3 * Y.std()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    W = 512
+    h, wrhf, ctx, orc = build("c1_small", W)
+    rng = np.random.default_rng(2)
+    ctx.set_walkers(perturbed_walkers(h, W, wrhf, rng, 0.02))
+    ctx.surrogate_init(hidden=128, seed=4, capacity=W * 24)
+    ctx.set_cv(subset_inv=16, risk_frac=0.05, seed=1)
+    for s in range(24):
+        grow_population(ctx, h, 1, rng)
+        ctx.measure()
+        ctx.surrogate_collect()
+    tr, va = ctx.surrogate_train(epochs=30, batch=256, lr=1e-3, seed=5)
+    return h, ctx, rng, tr, va
+
+
+def test_surrogate_on_walkers(trained):
+    h, ctx, rng, tr, va = trained
+    grow_population(ctx, h, 2, rng)
+    ctx.measure()
+    y = exact_y(ctx, h)
+    S = ctx.surrogate_predict()
+    w = ctx.weights()
+    live = w != 0
+    # held-out walkers outside the 10% tails of E_J
+    fb = ctx.force_bias()
+    ej = 0.5 * np.sum((fb[:, :h.nchol] ** 2).real, axis=1)
+    d = np.abs(ej - np.median(ej[live]))
+    tame = live & (d <= np.quantile(d[live], 0.9))
+    rms = np.sqrt(np.mean((S - y)[tame] ** 2))
+    sd = y[tame].std()
+    print("walkers: train rmse %.3f valid rmse %.3f | fresh tame rms %.3f  sd(y) %.3f  sd(E_K) n/a"
+          % (tr, va, rms, sd))
+    assert np.isfinite(S).all()
+    assert rms < 1.25 * sd
+
+
+def test_control_variate_exact_limits(trained):
+    h, ctx, rng, tr, va = trained
+    e_exact, _ = ctx.measure()
+    for kw in (dict(subset_inv=16, risk_frac=1.0), dict(subset_inv=1, risk_frac=0.0),
+               dict(subset_inv=1, risk_frac=0.05)):
+        ctx.set_cv(seed=3, **kw)
+        e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+        assert np.allclose(e, e_exact, rtol=0, atol=1e-8 * np.abs(e_exact).max()), kw
+    # fallback: a tiny threshold forces exact energies
+    ctx.set_cv(subset_inv=16, risk_frac=0.0, fallback_rms=1e-12, seed=7)
+    e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+    assert ctx.cv_stats()["fallback"] == 1.0
+    assert np.allclose(e, e_exact, rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("risk", [0.0, 0.05])
+def test_control_variate_unbiased(trained, risk):
+    h, ctx, rng, tr, va = trained
+    grow_population(ctx, h, 3, rng)
+    e_exact, _ = ctx.measure()
+    w = ctx.weights()
+    E0 = np.sum(w * e_exact) / np.sum(w)
+    ests, var_sub = [], []
+    for seed in range(40):
+        ctx.set_cv(subset_inv=16, risk_frac=risk, seed=100 + seed)
+        e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+        ests.append(np.sum(w * e) / np.sum(w))
+        var_sub.append(ctx.cv_stats()["var_sub"])
+    ests = np.array(ests)
+    sem = ests.std(ddof=1) / np.sqrt(len(ests))
+    print("risk %.2f exact %.8f cv mean %.8f sem %.2e  sd %.2e  predicted sd %.2e"
+          % (risk, E0, ests.mean(), sem, ests.std(ddof=1), np.sqrt(np.mean(var_sub))))
+    assert abs(ests.mean() - E0) < 4 * sem + 1e-9
+
+
+def test_control_variate_unbiased_with_untrained_surrogate():
+    W = 256
+    h, wrhf, ctx, orc = build("c1_small", W)
+    rng = np.random.default_rng(9)
+    ctx.set_walkers(perturbed_walkers(h, W, wrhf, rng, 0.02))
+    grow_population(ctx, h, 3, rng)
+    ctx.surrogate_init(hidden=32, seed=8)
+    p = ctx.surrogate_params()
+    p[-2], p[-1] = 0.0, 1.0
+    ctx.surrogate_params(p)  # marks it usable with random weights
+    e_exact, _ = ctx.measure()
+    w = ctx.weights()
+    ests = []
+    for seed in range(60):
+        ctx.set_cv(subset_inv=8, risk_frac=0.05, seed=seed)
+        e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+        ests.append(np.sum(w * e) / np.sum(w))
+    ests = np.array(ests)
+    sem = ests.std(ddof=1) / np.sqrt(len(ests))
+    assert abs(ests.mean() - np.sum(w * e_exact) / np.sum(w)) < 4 * sem

@@ -35,6 +35,7 @@ def lib():
         L.afq_create.argtypes = [C.POINTER(AfqHam), C.POINTER(AfqTrial), C.POINTER(AfqParams), C.POINTER(vp)]
         L.afq_destroy.argtypes = [vp]
         L.afq_set_nchol_ene.argtypes = [vp, i32]
+        L.afq_set_precision.argtypes = [vp, i32, i32, i32]
         L.afq_prep_prop.argtypes = [vp, DP, DP, DP, DP, d, d]
         L.afq_set_walkers.argtypes = [vp, DP, DP, DP]
         L.afq_get_walkers.argtypes = [vp, DP, DP, DP]
@@ -51,12 +52,24 @@ def lib():
         L.afq_sync.argtypes = [vp]
         L.afq_debug_gemm_f64.argtypes = [DP, DP, DP, i32, i32, i32]
         L.afq_debug_copy.argtypes = [i64]
+        FP = C.POINTER(C.c_float)
+        L.afq_surrogate_init.argtypes = [vp, i32, u64, i64]
+        L.afq_surrogate_collect.argtypes = [vp, C.POINTER(i64)]
+        L.afq_surrogate_load.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), i64, C.POINTER(i64)]
+        L.afq_surrogate_train.argtypes = [vp, i32, i32, d, u64, DP, DP]
+        L.afq_surrogate_predict.argtypes = [vp, DP]
+        L.afq_surrogate_params.argtypes = [vp, FP, i64, i32]
+        L.afq_surrogate_grad.argtypes = [vp, FP, FP, i32, FP, DP]
+        L.afq_set_cv.argtypes = [vp, i32, d, d, u64]
+        L.afq_cv_stats.argtypes = [vp, DP]
+        L.afq_energy_parts.argtypes = [vp, DP, DP]
+        L.afq_surrogate_features.argtypes = [vp, FP]
         L.afq_debug_gemm_bf16.argtypes = [DP, DP, C.POINTER(C.c_float), i32, i32, i32, i32]
         _lib = L
     return _lib
 
 
-STAGES = ["force_bias", "fields", "vhs", "propagate", "green", "weight", "ortho", "energy", "comb"]
+STAGES = ["force_bias", "fields", "vhs", "onebody", "green", "weight", "ortho", "exchange", "comb", "taylor"]
 
 
 def c2f(a):
@@ -125,6 +138,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_precision(self, prec_fb=AFQ_PREC_FP64, prec_vhs=AFQ_PREC_FP64, prec_exchange=AFQ_PREC_FP64):
+        self._chk(lib().afq_set_precision(self.ptr, prec_fb, prec_vhs, prec_exchange), "afq_set_precision")
 
     def set_nchol_ene(self, n):
         self._chk(lib().afq_set_nchol_ene(self.ptr, int(n)), "afq_set_nchol_ene")
@@ -201,12 +217,92 @@ class Context:
         return t[0]
 
     def stage_times(self, reset=True):
-        ms = np.zeros(9)
+        ms = np.zeros(10)
         self._chk(lib().afq_stage_times(self.ptr, dptr(ms), int(reset)), "afq_stage_times")
         return dict(zip(STAGES, ms))
 
     def sync(self):
         self._chk(lib().afq_sync(self.ptr), "afq_sync")
+
+    # ---- surrogate control variate
+    def surrogate_init(self, hidden=256, seed=1, capacity=0):
+        self._chk(lib().afq_surrogate_init(self.ptr, hidden, seed, capacity), "afq_surrogate_init")
+        self.hidden = hidden
+        self.nfeat = self.h.nchol + 2 * self.h.norb
+
+    def surrogate_collect(self):
+        n = C.c_int64(0)
+        self._chk(lib().afq_surrogate_collect(self.ptr, C.byref(n)), "afq_surrogate_collect")
+        return n.value
+
+    def surrogate_load(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        assert X.shape == (len(Y), self.nfeat)
+        n = C.c_int64(0)
+        fp = C.POINTER(C.c_float)
+        self._chk(lib().afq_surrogate_load(self.ptr, X.ctypes.data_as(fp), Y.ctypes.data_as(fp), len(Y),
+                                           C.byref(n)), "afq_surrogate_load")
+        return n.value
+
+    def surrogate_train(self, epochs=20, batch=512, lr=1e-3, seed=3):
+        a, b = C.c_double(0), C.c_double(0)
+        self._chk(lib().afq_surrogate_train(self.ptr, epochs, batch, lr, seed, C.byref(a), C.byref(b)),
+                  "afq_surrogate_train")
+        return a.value, b.value
+
+    def surrogate_predict(self):
+        out = np.zeros(self.nwalk)
+        self._chk(lib().afq_surrogate_predict(self.ptr, dptr(out)), "afq_surrogate_predict")
+        return out
+
+    def _nparams(self):
+        F, H = self.nfeat, self.hidden
+        dims = [F, H, H, H, 1]
+        return dims, sum(dims[l] * dims[l + 1] + dims[l + 1] for l in range(4)) + 2 * F + 2
+
+    def surrogate_params(self, new=None):
+        dims, n = self._nparams()
+        FP = C.POINTER(C.c_float)
+        if new is None:
+            buf = np.zeros(n, np.float32)
+            self._chk(lib().afq_surrogate_params(self.ptr, buf.ctypes.data_as(FP), n, 0), "afq_surrogate_params")
+            return buf
+        buf = np.ascontiguousarray(new, dtype=np.float32)
+        self._chk(lib().afq_surrogate_params(self.ptr, buf.ctypes.data_as(FP), len(buf), 1), "afq_surrogate_params")
+
+    def surrogate_grad(self, X, t):
+        dims, n = self._nparams()
+        FP = C.POINTER(C.c_float)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        ng = n - 2 * dims[0] - 2
+        g = np.zeros(ng, np.float32)
+        loss = C.c_double(0)
+        self._chk(lib().afq_surrogate_grad(self.ptr, X.ctypes.data_as(FP), t.ctypes.data_as(FP), len(t),
+                                           g.ctypes.data_as(FP), C.byref(loss)), "afq_surrogate_grad")
+        return loss.value, g
+
+    def energy_parts(self):
+        e1 = np.zeros(self.nwalk)
+        ek = np.zeros(self.nwalk)
+        self._chk(lib().afq_energy_parts(self.ptr, dptr(e1), dptr(ek)), "afq_energy_parts")
+        return e1, ek
+
+    def features(self):
+        F = self.h.nchol + 2 * self.h.norb
+        X = np.zeros((self.nwalk, F), np.float32)
+        self._chk(lib().afq_surrogate_features(self.ptr, X.ctypes.data_as(C.POINTER(C.c_float))),
+                  "afq_surrogate_features")
+        return X
+
+    def set_cv(self, subset_inv=16, risk_frac=0.05, fallback_rms=0.0, seed=5):
+        self._chk(lib().afq_set_cv(self.ptr, subset_inv, risk_frac, fallback_rms, seed), "afq_set_cv")
+
+    def cv_stats(self):
+        o = np.zeros(6)
+        self._chk(lib().afq_cv_stats(self.ptr, dptr(o)), "afq_cv_stats")
+        return dict(nsel=o[0], rms=o[1], fallback=o[2], var_sub=o[3], mean_resid=o[4], n_fallback=o[5])
 
 
 AFQ_RNG_REPLAY = 0
@@ -222,6 +318,9 @@ class RunParams(C.Structure):
         ("max_steps", C.c_int32), ("rng_mode", C.c_int32), ("warmup_steps", C.c_int32),
         ("prec_fb", C.c_int32), ("prec_vhs", C.c_int32), ("prec_exchange", C.c_int32),
         ("energy_mode", C.c_int32), ("device", C.c_int32),
+        ("surrogate_train_steps", C.c_int32), ("surrogate_hidden", C.c_int32),
+        ("surrogate_epochs", C.c_int32), ("subset_inv", C.c_int32), ("fallback_rms", C.c_double),
+        ("risk_frac", C.c_double),
     ]
 
 
@@ -229,7 +328,9 @@ class RunResult(C.Structure):
     _fields_ = [("ref_energy", C.c_double), ("nchol_ene", C.c_int32), ("delta", C.c_double),
                 ("n_large_deviations", C.c_int64), ("prop_time", C.c_double),
                 ("ene_time", C.c_double), ("timed_steps", C.c_int64), ("timed_wall_s", C.c_double),
-                ("stage_ms", C.c_double * 9)]
+                ("stage_ms", C.c_double * 10), ("train_rmse", C.c_double), ("valid_rmse", C.c_double),
+                ("cv_var_sub", C.c_double), ("cv_rms", C.c_double), ("cv_fallbacks", C.c_int64),
+                ("train_samples", C.c_int64), ("est_var", C.c_double)]
 
 
 def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, nsweeps, ortho_steps,
@@ -237,7 +338,8 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   burn_iter=0, ene0_guess=1e10, weight_cap=0.0, max_steps=0,
                   rng_mode=AFQ_RNG_REPLAY, warmup_steps=0, prec_fb=AFQ_PREC_FP64,
                   prec_vhs=AFQ_PREC_FP64, prec_exchange=AFQ_PREC_FP64,
-                  energy_mode=AFQ_ENERGY_EXACT, device=0):
+                  energy_mode=AFQ_ENERGY_EXACT, device=0, surrogate_train_steps=0, surrogate_hidden=256,
+                  surrogate_epochs=20, subset_inv=16, fallback_rms=0.0, risk_frac=0.05):
     """calcMixedEstimatorLongProp on the GPU; returns (weights, energies, RunResult)."""
     L = lib()
     if not hasattr(L, "_run_bound"):
@@ -247,7 +349,9 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
     hs, ts = ham.ham_struct(), ham.trial_struct()
     p = RunParams(seed, dt, nsteps, nsweeps, ortho_steps, burn_iter, nwalk, nranks, int(walker_rhf),
                   int(ref_compat), cholesky_threshold, ene0_guess, weight_cap, max_steps, rng_mode,
-                  warmup_steps, prec_fb, prec_vhs, prec_exchange, energy_mode, device)
+                  warmup_steps, prec_fb, prec_vhs, prec_exchange, energy_mode, device,
+                  surrogate_train_steps, surrogate_hidden, surrogate_epochs, subset_inv, fallback_rms,
+                  risk_frac)
     w = np.zeros(nsweeps)
     e = np.zeros(nsweeps)
     res = RunResult()

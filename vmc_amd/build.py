@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvmc_afqmc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["afq_kernels.hip", "afq_ctx.cpp", "afq_driver.cpp"]
+SOURCES = ["afq_kernels.hip", "afq_mlp.hip", "afq_ctx.cpp", "afq_driver.cpp"]
 HEADERS = ["afq_device.hpp", "afq_internal.hpp", "afq_driver.hpp"]
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
+#include <random>
 #include <complex>
 #include <cstring>
 #include <stdexcept>
@@ -95,6 +97,20 @@ struct afq_ctx {
   std::vector<EvPair> pending, pool;
   double stage_ms[afq::ST_COUNT] = {0};
   bool timing = true;
+  // surrogate control variate
+  afq::Mlp mlp;
+  int cv_minv = 16;
+  double cv_risk = 0.05;
+  double cv_fallback = 0.0;
+  std::mt19937_64 cv_rng{1};
+  double* cvS = nullptr;
+  double* cvEK = nullptr;
+  double* cvEJ = nullptr;
+  float* cvPinv = nullptr;
+  int* cvSel = nullptr;
+  float* cvX = nullptr;
+  double cv_stats[6] = {0, 0, 0, 0, 0, 0};
+  bool last_exact_all = false;
 
   hipEvent_t ev_get() {
     hipEvent_t e;
@@ -155,6 +171,37 @@ void free_all(DevState& s) {
     if (p) (void)hipFree(p);
   if (s.idx) (void)hipFree(s.idx);
   if (s.ncap) (void)hipFree(s.ncap);
+  unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo};
+  for (unsigned short* p : hp)
+    if (p) (void)hipFree(p);
+}
+
+// bf16 operand copies for the low-precision exchange (allocated on first use)
+void ensure_lowprec(DevState& s, hipStream_t st) {
+  if (s.Rb_hi) return;
+  int64_t rows = 0;
+  for (int q = 0; q < s.nslots; ++q) rows += (int64_t)s.G * s.slot[q].n;
+  s.Rb_hi = dalloc<unsigned short>(rows * s.Np);
+  s.Rb_lo = dalloc<unsigned short>(rows * s.Np);
+  s.Tb_hi = dalloc<unsigned short>((int64_t)s.W * s.ctot * s.Np);
+  s.Tb_lo = dalloc<unsigned short>((int64_t)s.W * s.ctot * s.Np);
+  afq::launch_rsplit(s, st);
+  launch_check("rsplit");
+}
+
+bool exchange_prec_ok(int p) {
+  return p == AFQ_PREC_FP64 || p == AFQ_PREC_BF16X3 || p == AFQ_PREC_BF16;
+}
+
+// Frees the walker buffers of a temporary population that shares the
+// Hamiltonian, trial and bf16 operand buffers of its context.
+void free_all(DevState& s);
+void free_temporary(const DevState& t) {
+  DevState z = t;
+  z.E = z.Lp = z.R = z.mf = nullptr;
+  z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
+  z.Rb_hi = z.Rb_lo = z.Tb_hi = z.Tb_lo = nullptr;
+  free_all(z);
 }
 
 // Walker-dependent buffers for a population of W.
@@ -288,7 +335,43 @@ void refresh_walkers(afq_ctx* c, bool overlaps_from_dets) {
   launch_check("force_bias");
 }
 
+void free_mlp(afq_ctx* c) {
+  afq::Mlp& m = c->mlp;
+  std::vector<void*> ptrs = {m.mu, m.sd, m.lossbuf, m.X, m.Y, c->cvS, c->cvEK, c->cvEJ, c->cvPinv, c->cvSel, c->cvX};
+  for (int l = 0; l < 4; ++l)
+    for (void* p : {(void*)m.w[l], (void*)m.b[l], (void*)m.mw[l], (void*)m.vw[l], (void*)m.mb[l], (void*)m.vb[l],
+                    (void*)m.gw[l], (void*)m.gb[l], (void*)m.z[l], (void*)m.h[l], (void*)m.dh[l]})
+      ptrs.push_back(p);
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  m = afq::Mlp();
+  c->cvS = c->cvEK = c->cvEJ = nullptr;
+  c->cvPinv = nullptr;
+  c->cvSel = nullptr;
+  c->cvX = nullptr;
+}
+
+// synchronous copy ordered after all work queued on the context's (non-blocking) stream
+hipError_t scopy(afq_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(c->stream);
+}
+
+int64_t mlp_nparams(const afq::Mlp& m) {
+  int64_t n = 0;
+  for (int l = 0; l < 4; ++l) n += (int64_t)m.dims[l] * m.dims[l + 1] + m.dims[l + 1];
+  return n;
+}
+
 }  // namespace
+
+// Driver support: keep a context's last error readable after the context is gone.
+__attribute__((visibility("hidden"))) void afq_keep_error(const afq_ctx* c, int rc) {
+  if (c && rc != AFQ_OK) g_create_error = c->err;
+}
 
 extern "C" {
 
@@ -312,7 +395,7 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
   if (na > 64 || nb > 64)
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "more than 64 electrons per spin not supported by this build");
   if (params->prec_fb != AFQ_PREC_FP64 || params->prec_vhs != AFQ_PREC_FP64 ||
-      params->prec_exchange != AFQ_PREC_FP64)
+      !exchange_prec_ok(params->prec_exchange))
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
   afq_ctx* c = new afq_ctx();
   try {
@@ -342,6 +425,10 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     }
     s.Ktot = 0;
     for (int q = 0; q < s.nslots; ++q) s.Ktot += (int64_t)N * s.slot[q].n;
+    s.Np = (N + 7) / 8 * 8;
+    s.ctot = 0;
+    for (int q = 0; q < s.nslots; ++q) s.ctot += 2 * s.slot[q].n;
+    s.prec_exchange = params->prec_exchange;
     int ngt = 0;
     for (int q = 0; q < s.nslots; ++q)
       ngt = std::max(ngt, (G + afq::exchange_tile_g(s.slot[q].n) - 1) / afq::exchange_tile_g(s.slot[q].n));
@@ -384,6 +471,7 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     s.mf = dalloc<double>(2 * (size_t)G);
     alloc_walkers(s);
     afq::launch_fill(s.weights, W, 0.0, c->stream);
+    if (s.prec_exchange != AFQ_PREC_FP64) ensure_lowprec(s, c->stream);
     hip_check(hipStreamSynchronize(c->stream), "setup sync");
   } catch (const DevError& e) {
     free_all(c->s);
@@ -404,9 +492,22 @@ int afq_destroy(afq_ctx* c) {
   for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto& p : c->pool) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   free_all(c->s);
+  free_mlp(c);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return AFQ_OK;
+}
+
+int afq_set_precision(afq_ctx* c, int32_t prec_fb, int32_t prec_vhs, int32_t prec_exchange) {
+  if (!c) return AFQ_ERR_ARG;
+  if (prec_fb != AFQ_PREC_FP64 || prec_vhs != AFQ_PREC_FP64 || !exchange_prec_ok(prec_exchange))
+    return fail(c, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
+  AFQ_GUARD_BEGIN
+  if (prec_exchange != AFQ_PREC_FP64) ensure_lowprec(c->s, c->stream);
+  c->s.prec_exchange = prec_exchange;
+  c->params.prec_exchange = prec_exchange;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
 }
 
 int afq_set_nchol_ene(afq_ctx* c, int32_t n) {
@@ -514,6 +615,7 @@ int afq_propagate(afq_ctx* c, const double* draws, uint64_t seed, uint64_t step,
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
   afq::launch_zero_counter(s.ncap, c->stream);
+  c->last_exact_all = false;
   if (draws) upload(s.draws, draws, (size_t)s.W * s.G, c->stream);
   {
     Stage t(c, afq::ST_FIELDS);
@@ -527,7 +629,15 @@ int afq_propagate(afq_ctx* c, const double* draws, uint64_t seed, uint64_t step,
   }
   {
     Stage t(c, afq::ST_PROP);
-    afq::launch_propagate(s, c->stream);
+    afq::launch_expmul_pre(s, c->stream);
+  }
+  {
+    Stage t(c, afq::ST_TAYLOR);
+    afq::launch_taylor(s, c->stream);
+  }
+  {
+    Stage t(c, afq::ST_PROP);
+    afq::launch_expmul_post(s, c->stream);
     launch_check("propagate");
   }
   {
@@ -566,18 +676,136 @@ int afq_orthogonalize(afq_ctx* c) {
   AFQ_GUARD_END(c)
 }
 
+// Risk split of the live walkers: the ones whose Coulomb energy E_J lies
+// farthest from the median (the fraction cv_risk of them) carry the heavy tail of
+// the local-energy distribution; they are always evaluated exactly.  Returns the
+// live flags and the risk flags.
+static void risk_split(afq_ctx* c, std::vector<char>& live, std::vector<char>& risky) {
+  DevState& s = c->s;
+  std::vector<double> ej(s.W), w(s.W);
+  afq::launch_ej(s, c->cvEJ, c->stream);
+  launch_check("ej");
+  download(ej.data(), c->cvEJ, s.W, c->stream);
+  download(w.data(), s.weights, s.W, c->stream);
+  live.assign(s.W, 0);
+  risky.assign(s.W, 0);
+  std::vector<double> lv;
+  for (int i = 0; i < s.W; ++i)
+    if (w[i] != 0.0) { live[i] = 1; lv.push_back(ej[i]); }
+  const int nl = (int)lv.size();
+  const int nr = std::min(nl, (int)std::ceil(c->cv_risk * nl));
+  if (nl == 0 || nr == 0) return;
+  std::nth_element(lv.begin(), lv.begin() + nl / 2, lv.end());
+  const double med = lv[nl / 2];
+  std::vector<std::pair<double, int>> d;
+  for (int i = 0; i < s.W; ++i)
+    if (live[i]) d.push_back({-std::fabs(ej[i] - med), i});
+  std::partial_sort(d.begin(), d.begin() + nr, d.end());
+  for (int k = 0; k < nr; ++k) risky[d[k].second] = 1;
+}
+
+// Two-level control variate.  The surrogate predicts y = E_K - E_J on every
+// walker; the exact exchange is evaluated on (a) every walker of the risk
+// stratum (inclusion probability 1) and (b) one walker drawn uniformly from each
+// consecutive block of cv_minv remaining live walkers (inclusion probability
+// 1/b for a block of b).  The Horvitz-Thompson correction keeps every walker's
+// estimate unbiased for its exact local energy.  Every walker is evaluated
+// exactly instead when the residual RMS of the sample exceeds the fallback
+// threshold.
+static int measure_cv(afq_ctx* c, double delta, double* eloc) {
+  DevState& s = c->s;
+  afq::Mlp& m = c->mlp;
+  if (!m.ready) return fail(c, AFQ_ERR_STATE, "surrogate not trained");
+  const int mm = c->cv_minv;
+  afq::launch_features(s, nullptr, s.W, c->cvX, 0, c->stream);
+  afq::launch_predict(s, m, c->cvX, s.W, c->cvS, c->stream);
+  std::vector<char> live, risky;
+  risk_split(c, live, risky);
+  std::vector<int> sel, tame;
+  std::vector<float> pinv(s.W, 0.f);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  for (int i = 0; i < s.W; ++i) {
+    if (risky[i]) { sel.push_back(i); pinv[i] = 1.f; }
+    else if (live[i]) tame.push_back(i);
+  }
+  const int nrisk = (int)sel.size();
+  for (size_t b0 = 0; b0 < tame.size(); b0 += mm) {
+    const int b = (int)std::min<size_t>(mm, tame.size() - b0);
+    const int k = tame[b0 + std::min(b - 1, (int)(U(c->cv_rng) * b))];
+    sel.push_back(k);
+    pinv[k] = (float)b;
+  }
+  const int nsel = (int)sel.size();
+  hip_check(hipMemcpyAsync(c->cvPinv, pinv.data(), s.W * sizeof(float), hipMemcpyHostToDevice, c->stream), "H2D");
+  if (nsel > 0)
+    hip_check(hipMemcpyAsync(c->cvSel, sel.data(), nsel * sizeof(int), hipMemcpyHostToDevice, c->stream), "H2D");
+  if (nsel > 0 && s.GE > 0) {
+    Stage t(c, afq::ST_ENERGY);
+    if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, c->cvSel, nsel, c->stream);
+    else afq::launch_exchange_lp(s, c->cvSel, nsel, s.prec_exchange, c->stream);
+    launch_check("exchange (subset)");
+  }
+  if (nsel > 0) afq::launch_ek(s, c->cvSel, nsel, c->cvEK, nullptr, 0, c->stream);
+  afq::launch_energy_cv(s, c->cvS, c->cvEK, c->cvPinv, delta, c->stream);
+  launch_check("energy_cv");
+  // residual statistics of the random part of the sample
+  std::vector<double> S(s.W), EK(s.W), EJ(s.W), w(s.W);
+  download(S.data(), c->cvS, s.W, c->stream);
+  download(EK.data(), c->cvEK, s.W, c->stream);
+  download(EJ.data(), c->cvEJ, s.W, c->stream);
+  download(w.data(), s.weights, s.W, c->stream);
+  double r2 = 0.0, rm = 0.0, wr2 = 0.0, wsum = 0.0;
+  for (int i = 0; i < s.W; ++i) wsum += w[i];
+  for (int q = nrisk; q < nsel; ++q) {
+    const int k = sel[q];
+    const double r = S[k] - (EK[k] - EJ[k]);
+    r2 += r * r;
+    rm += r;
+    wr2 += (double)pinv[k] * (pinv[k] - 1.0) * (w[k] * r) * (w[k] * r);
+  }
+  const int nrand = nsel - nrisk;
+  const double rms = nrand ? std::sqrt(r2 / nrand) : 0.0;
+  bool fallback = false;
+  if (c->cv_fallback > 0.0 && rms > c->cv_fallback) {
+    fallback = true;
+    if (s.GE > 0) {
+      Stage t(c, afq::ST_ENERGY);
+      if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, nullptr, s.W, c->stream);
+      else afq::launch_exchange_lp(s, nullptr, s.W, s.prec_exchange, c->stream);
+    }
+    afq::launch_energy(s, nullptr, s.W, delta, c->stream);
+    launch_check("fallback exact");
+  }
+  c->cv_stats[0] = nsel;
+  c->cv_stats[1] = rms;
+  c->cv_stats[2] = fallback ? 1.0 : 0.0;
+  c->cv_stats[3] = fallback || wsum <= 0 ? 0.0 : wr2 / (wsum * wsum);
+  c->cv_stats[4] = nrand ? rm / nrand : 0.0;
+  c->cv_stats[5] += fallback ? 1.0 : 0.0;
+  c->last_exact_all = fallback;
+  if (eloc) download(eloc, s.eloc, s.W, c->stream);
+  return AFQ_OK;
+}
+
 int afq_measure(afq_ctx* c, int32_t mode, double delta, double* eloc, double* ham_ovlp) {
   if (!c) return AFQ_ERR_ARG;
-  if (mode != AFQ_ENERGY_EXACT) return fail(c, AFQ_ERR_UNSUPPORTED, "energy mode not available");
+  if (mode != AFQ_ENERGY_EXACT && mode != AFQ_ENERGY_SURROGATE)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "energy mode not available");
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
+  if (mode == AFQ_ENERGY_SURROGATE) {
+    if (ham_ovlp) return fail(c, AFQ_ERR_ARG, "ham_ovlp is not produced by the surrogate mode");
+    return measure_cv(c, delta, eloc);
+  }
   if (s.GE > 0) {
     Stage t(c, afq::ST_ENERGY);  // the exchange contraction alone
-    afq::launch_exchange(s, nullptr, s.W, c->stream);
+    if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, nullptr, s.W, c->stream);
+    else afq::launch_exchange_lp(s, nullptr, s.W, s.prec_exchange, c->stream);
     launch_check("exchange");
   }
   afq::launch_energy(s, nullptr, s.W, delta, c->stream);
   launch_check("energy");
+  c->last_exact_all = true;
   if (eloc) download(eloc, s.eloc, s.W, c->stream);
   if (ham_ovlp) download(ham_ovlp, s.ho, 4 * (size_t)s.W, c->stream);
   return AFQ_OK;
@@ -604,16 +832,10 @@ int afq_ham_and_overlap(afq_ctx* c, const double* dets, int32_t ndet, int32_t nc
     launch_check("ham_and_overlap");
     download(out, t.ho, 4 * (size_t)ndet, c->stream);
   } catch (...) {
-    DevState z = t;
-    z.E = z.Lp = z.R = z.mf = nullptr;
-    z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
-    free_all(z);
+    free_temporary(t);
     throw;
   }
-  DevState z = t;
-  z.E = z.Lp = z.R = z.mf = nullptr;
-  z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
-  free_all(z);
+  free_temporary(t);
   return AFQ_OK;
   AFQ_GUARD_END(c)
 }
@@ -641,12 +863,12 @@ int afq_comb(afq_ctx* c, double zeta, double* total_weight) {
   AFQ_GUARD_END(c)
 }
 
-int afq_stage_times(afq_ctx* c, double* ms9, int32_t reset) {
+int afq_stage_times(afq_ctx* c, double* ms10, int32_t reset) {
   if (!c) return AFQ_ERR_ARG;
   AFQ_GUARD_BEGIN
   c->resolve();
   for (int i = 0; i < afq::ST_COUNT; ++i) {
-    if (ms9) ms9[i] = c->stage_ms[i];
+    if (ms10) ms10[i] = c->stage_ms[i];
     if (reset) c->stage_ms[i] = 0.0;
   }
   return AFQ_OK;
@@ -707,6 +929,337 @@ int afq_debug_copy(int64_t bytes) {
   } catch (const DevError& e) {
     return fail(nullptr, e.code, e.what());
   }
+  return AFQ_OK;
+}
+
+// ------------------------------------------------------------------ surrogate
+int afq_surrogate_init(afq_ctx* c, int32_t hidden, uint64_t seed, int64_t capacity) {
+  if (!c || hidden <= 0 || capacity < 0) return c ? fail(c, AFQ_ERR_ARG, "bad surrogate arguments") : AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  free_mlp(c);
+  DevState& s = c->s;
+  afq::Mlp& m = c->mlp;
+  const int F = s.G + 2 * s.N;
+  m.dims[0] = F; m.dims[1] = hidden; m.dims[2] = hidden; m.dims[3] = hidden; m.dims[4] = 1;
+  m.Bcap = std::max(s.W, 4096);
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l < 4; ++l) {
+    const int64_t nw = (int64_t)m.dims[l] * m.dims[l + 1], nb = m.dims[l + 1];
+    std::vector<float> wi(nw), bi(nb, 0.f);
+    // uniform(-a, a), a = sqrt(6 / (fan_in + fan_out))
+    const double a = std::sqrt(6.0 / (m.dims[l] + m.dims[l + 1]));
+    std::uniform_real_distribution<double> U(-a, a);
+    for (auto& x : wi) x = (float)U(rng);
+    float** bufs[] = {&m.w[l], &m.mw[l], &m.vw[l], &m.gw[l]};
+    for (float** b : bufs) *b = dalloc<float>(nw);
+    float** bbufs[] = {&m.b[l], &m.mb[l], &m.vb[l], &m.gb[l]};
+    for (float** b : bbufs) *b = dalloc<float>(nb);
+    hip_check(scopy(c, m.w[l], wi.data(), nw * 4, hipMemcpyHostToDevice), "H2D");
+    hip_check(scopy(c, m.b[l], bi.data(), nb * 4, hipMemcpyHostToDevice), "H2D");
+    std::vector<float> zw(nw, 0.f), zb(nb, 0.f);
+    hip_check(scopy(c, m.mw[l], zw.data(), nw * 4, hipMemcpyHostToDevice), "H2D");
+    hip_check(scopy(c, m.vw[l], zw.data(), nw * 4, hipMemcpyHostToDevice), "H2D");
+    hip_check(scopy(c, m.mb[l], zb.data(), nb * 4, hipMemcpyHostToDevice), "H2D");
+    hip_check(scopy(c, m.vb[l], zb.data(), nb * 4, hipMemcpyHostToDevice), "H2D");
+    m.z[l] = dalloc<float>((int64_t)m.Bcap * m.dims[l + 1]);
+    m.h[l] = dalloc<float>((int64_t)m.Bcap * m.dims[l + 1]);
+    m.dh[l] = dalloc<float>((int64_t)m.Bcap * m.dims[l + 1]);
+  }
+  m.mu = dalloc<float>(F);
+  m.sd = dalloc<float>(F);
+  std::vector<float> z0(F, 0.f), o1(F, 1.f);
+  hip_check(scopy(c, m.mu, z0.data(), F * 4, hipMemcpyHostToDevice), "H2D");
+  hip_check(scopy(c, m.sd, o1.data(), F * 4, hipMemcpyHostToDevice), "H2D");
+  m.lossbuf = dalloc<float>(1);
+  m.cap = capacity;
+  if (capacity > 0) {
+    m.X = dalloc<float>(capacity * F);
+    m.Y = dalloc<float>(capacity);
+  }
+  c->cvS = dalloc<double>(s.W);
+  c->cvEK = dalloc<double>(s.W);
+  c->cvEJ = dalloc<double>(s.W);
+  c->cvPinv = dalloc<float>(s.W);
+  c->cvSel = dalloc<int>(s.W);
+  c->cvX = dalloc<float>((int64_t)s.W * F);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_surrogate_collect(afq_ctx* c, int64_t* total) {
+  if (!c || !c->mlp.dims[0]) return c ? fail(c, AFQ_ERR_STATE, "surrogate not initialised") : AFQ_ERR_ARG;
+  if (!c->last_exact_all) return fail(c, AFQ_ERR_STATE, "collect needs an exact measurement of every walker");
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  afq::Mlp& m = c->mlp;
+  // the risk stratum is always evaluated exactly, so the surrogate learns the rest
+  std::vector<char> lv, risky;
+  risk_split(c, lv, risky);
+  std::vector<int> live;
+  for (int i = 0; i < s.W; ++i)
+    if (lv[i] && !risky[i]) live.push_back(i);
+  const int64_t room = m.cap - m.n;
+  const int nadd = (int)std::min<int64_t>(room, (int64_t)live.size());
+  if (nadd > 0) {
+    hip_check(hipMemcpyAsync(c->cvSel, live.data(), nadd * sizeof(int), hipMemcpyHostToDevice, c->stream), "H2D");
+    afq::launch_features(s, c->cvSel, nadd, m.X, m.n, c->stream);
+    afq::launch_ek(s, c->cvSel, nadd, nullptr, m.Y, m.n, c->stream);
+    launch_check("collect");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    m.n += nadd;
+  }
+  if (total) *total = m.n;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_surrogate_load(afq_ctx* c, const float* X, const float* Y, int64_t n, int64_t* total) {
+  if (!c || !X || !Y || n < 0) return c ? fail(c, AFQ_ERR_ARG, "bad arguments") : AFQ_ERR_ARG;
+  afq::Mlp& m = c->mlp;
+  if (!m.dims[0]) return fail(c, AFQ_ERR_STATE, "surrogate not initialised");
+  if (m.n + n > m.cap) return fail(c, AFQ_ERR_ARG, "training set capacity exceeded");
+  AFQ_GUARD_BEGIN
+  const int F = m.dims[0];
+  hip_check(scopy(c, m.X + m.n * F, X, (size_t)n * F * 4, hipMemcpyHostToDevice), "H2D");
+  hip_check(scopy(c, m.Y + m.n, Y, (size_t)n * 4, hipMemcpyHostToDevice), "H2D");
+  m.n += n;
+  if (total) *total = m.n;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_surrogate_train(afq_ctx* c, int32_t epochs, int32_t batch, double lr, uint64_t seed, double* train_rmse,
+                        double* valid_rmse) {
+  if (!c || !c->mlp.dims[0]) return c ? fail(c, AFQ_ERR_STATE, "surrogate not initialised") : AFQ_ERR_ARG;
+  afq::Mlp& m = c->mlp;
+  if (m.n < 16) return fail(c, AFQ_ERR_STATE, "not enough training samples");
+  if (batch <= 0 || batch > m.Bcap) return fail(c, AFQ_ERR_ARG, "bad batch size");
+  AFQ_GUARD_BEGIN
+  const int F = m.dims[0];
+  const int64_t n = m.n;
+  // standardisation from the data
+  std::vector<float> X((size_t)n * F), Y(n);
+  hip_check(scopy(c, X.data(), m.X, X.size() * 4, hipMemcpyDeviceToHost), "D2H");
+  hip_check(scopy(c, Y.data(), m.Y, Y.size() * 4, hipMemcpyDeviceToHost), "D2H");
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::mt19937_64 rng(seed);
+  std::shuffle(perm.begin(), perm.end(), rng);
+  const int64_t nval = std::max<int64_t>(1, n / 10), ntr = n - nval;
+  std::vector<double> mu(F, 0.0), var(F, 0.0);
+  double ym = 0.0, yv = 0.0;
+  for (int64_t q = 0; q < ntr; ++q) {
+    const float* x = &X[(size_t)perm[q] * F];
+    for (int f = 0; f < F; ++f) mu[f] += x[f];
+    ym += Y[perm[q]];
+  }
+  for (int f = 0; f < F; ++f) mu[f] /= ntr;
+  ym /= ntr;
+  for (int64_t q = 0; q < ntr; ++q) {
+    const float* x = &X[(size_t)perm[q] * F];
+    for (int f = 0; f < F; ++f) var[f] += (x[f] - mu[f]) * (x[f] - mu[f]);
+    yv += (Y[perm[q]] - ym) * (Y[perm[q]] - ym);
+  }
+  std::vector<float> muf(F), sdf(F);
+  for (int f = 0; f < F; ++f) {
+    muf[f] = (float)mu[f];
+    const double sdv = std::sqrt(var[f] / ntr);
+    sdf[f] = (float)(sdv > 1e-8 ? sdv : 1.0);
+  }
+  hip_check(scopy(c, m.mu, muf.data(), F * 4, hipMemcpyHostToDevice), "H2D");
+  hip_check(scopy(c, m.sd, sdf.data(), F * 4, hipMemcpyHostToDevice), "H2D");
+  m.t_mu = (float)ym;
+  m.t_sd = (float)std::max(std::sqrt(yv / ntr), 1e-8);
+  // didx[0, ntr): training indices (reshuffled every epoch); didx[ntr, n): validation
+  int* didx = dalloc<int>(n);
+  std::vector<float*> best;
+  auto release = [&]() {
+    (void)hipFree(didx);
+    for (float* p : best) (void)hipFree(p);
+  };
+  try {
+    hip_check(scopy(c, didx, perm.data(), n * sizeof(int), hipMemcpyHostToDevice), "H2D");
+    // RMSE (Ha) of the prediction over cnt samples listed at idx (device)
+    auto rmse = [&](const int* idx, int64_t cnt) {
+      double acc = 0.0;
+      std::vector<float> y(std::min<int64_t>(cnt, m.Bcap));
+      std::vector<int> ih(cnt);
+      hip_check(scopy(c, ih.data(), idx, cnt * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+      for (int64_t b0 = 0; b0 < cnt; b0 += m.Bcap) {
+        const int B = (int)std::min<int64_t>(m.Bcap, cnt - b0);
+        afq::mlp_forward(m, m.X, idx + b0, B, c->stream);
+        hip_check(hipMemcpyAsync(y.data(), m.h[3], B * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        for (int q = 0; q < B; ++q) {
+          const double pred = (double)y[q] * m.t_sd + m.t_mu;
+          const double d = pred - Y[ih[b0 + q]];
+          acc += d * d;
+        }
+      }
+      return std::sqrt(acc / cnt);
+    };
+    // the weights of the epoch with the lowest validation error are kept
+    auto snapshot = [&](bool restore) {
+      for (int l = 0, k = 0; l < 4; ++l)
+        for (int wb = 0; wb < 2; ++wb, ++k) {
+          const size_t bytes = 4 * (size_t)(wb == 0 ? m.dims[l] * m.dims[l + 1] : m.dims[l + 1]);
+          float* live = wb == 0 ? m.w[l] : m.b[l];
+          if ((int)best.size() <= k) best.push_back(dalloc<float>(bytes / 4));
+          hip_check(hipMemcpyAsync(restore ? live : best[k], restore ? best[k] : live, bytes,
+                                   hipMemcpyDeviceToDevice, c->stream), "D2D");
+        }
+    };
+    double best_va = rmse(didx + ntr, nval);
+    snapshot(false);
+    std::vector<int> tr(perm.begin(), perm.begin() + ntr);
+    for (int ep = 0; ep < epochs; ++ep) {
+      std::shuffle(tr.begin(), tr.end(), rng);
+      hip_check(hipMemcpyAsync(didx, tr.data(), ntr * sizeof(int), hipMemcpyHostToDevice, c->stream), "H2D");
+      for (int64_t b0 = 0; b0 + batch <= ntr; b0 += batch) {
+        afq::mlp_forward(m, m.X, didx + b0, batch, c->stream);
+        afq::mlp_loss(m, m.Y, didx + b0, batch, c->stream);
+        afq::mlp_backward(m, m.X, didx + b0, batch, c->stream);
+        afq::mlp_adam(m, (float)lr, c->stream);
+      }
+      launch_check("train");
+      const double va = rmse(didx + ntr, nval);
+      if (va < best_va) {
+        best_va = va;
+        snapshot(false);
+      }
+    }
+    snapshot(true);
+    if (train_rmse) *train_rmse = rmse(didx, ntr);
+    if (valid_rmse) *valid_rmse = best_va;
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
+  m.ready = true;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_surrogate_predict(afq_ctx* c, double* pred) {
+  if (!c || !c->mlp.dims[0]) return c ? fail(c, AFQ_ERR_STATE, "surrogate not initialised") : AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  afq::launch_features(s, nullptr, s.W, c->cvX, 0, c->stream);
+  afq::launch_predict(s, c->mlp, c->cvX, s.W, c->cvS, c->stream);
+  launch_check("predict");
+  if (pred) download(pred, c->cvS, s.W, c->stream);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_surrogate_params(afq_ctx* c, float* buf, int64_t nbuf, int32_t set) {
+  if (!c || !c->mlp.dims[0]) return c ? fail(c, AFQ_ERR_STATE, "surrogate not initialised") : AFQ_ERR_ARG;
+  afq::Mlp& m = c->mlp;
+  const int F = m.dims[0];
+  const int64_t need = mlp_nparams(m) + 2 * F + 2;
+  if (!buf) return (int)AFQ_ERR_ARG;
+  if (nbuf < need) return fail(c, AFQ_ERR_ARG, "buffer too small: need " + std::to_string(need));
+  AFQ_GUARD_BEGIN
+  int64_t off = 0;
+  auto xfer = [&](float* d, int64_t cnt) {
+    if (set) hip_check(scopy(c, d, buf + off, cnt * 4, hipMemcpyHostToDevice), "H2D");
+    else hip_check(scopy(c, buf + off, d, cnt * 4, hipMemcpyDeviceToHost), "D2H");
+    off += cnt;
+  };
+  for (int l = 0; l < 4; ++l) {
+    xfer(m.w[l], (int64_t)m.dims[l] * m.dims[l + 1]);
+    xfer(m.b[l], m.dims[l + 1]);
+  }
+  xfer(m.mu, F);
+  xfer(m.sd, F);
+  if (set) { m.t_mu = buf[off]; m.t_sd = buf[off + 1]; m.ready = true; }
+  else { buf[off] = m.t_mu; buf[off + 1] = m.t_sd; }
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_surrogate_grad(afq_ctx* c, const float* X, const float* t, int32_t B, float* grads, double* loss) {
+  if (!c || !c->mlp.dims[0]) return c ? fail(c, AFQ_ERR_STATE, "surrogate not initialised") : AFQ_ERR_ARG;
+  afq::Mlp& m = c->mlp;
+  if (!X || !t || B <= 0 || B > m.Bcap) return fail(c, AFQ_ERR_ARG, "bad batch");
+  AFQ_GUARD_BEGIN
+  const int F = m.dims[0];
+  float* dX = dalloc<float>((int64_t)B * F);
+  float* dT = dalloc<float>(B);
+  hip_check(scopy(c, dX, X, (size_t)B * F * 4, hipMemcpyHostToDevice), "H2D");
+  hip_check(scopy(c, dT, t, (size_t)B * 4, hipMemcpyHostToDevice), "H2D");
+  afq::mlp_forward(m, dX, nullptr, B, c->stream);
+  const float l = afq::mlp_loss(m, dT, nullptr, B, c->stream);
+  afq::mlp_backward(m, dX, nullptr, B, c->stream);
+  launch_check("grad");
+  int64_t off = 0;
+  for (int l2 = 0; l2 < 4; ++l2) {
+    const int64_t nw = (int64_t)m.dims[l2] * m.dims[l2 + 1];
+    hip_check(scopy(c, grads + off, m.gw[l2], nw * 4, hipMemcpyDeviceToHost), "D2H");
+    off += nw;
+    hip_check(scopy(c, grads + off, m.gb[l2], m.dims[l2 + 1] * 4, hipMemcpyDeviceToHost), "D2H");
+    off += m.dims[l2 + 1];
+  }
+  if (loss) *loss = l;
+  (void)hipFree(dX);
+  (void)hipFree(dT);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_energy_parts(afq_ctx* c, double* e1, double* ek) {
+  if (!c || !c->last_exact_all) return c ? fail(c, AFQ_ERR_STATE, "needs an exact measurement of every walker") : AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  if (e1) {
+    std::vector<double> v(2 * (size_t)s.W * s.nslots);
+    download(v.data(), s.e1, v.size(), c->stream);
+    for (int w = 0; w < s.W; ++w) {
+      double a = 0.0;
+      for (int q = 0; q < s.nslots; ++q) a += s.slot[q].mult * v[2 * ((size_t)w * s.nslots + q)];
+      e1[w] = a;
+    }
+  }
+  if (ek) {
+    double* d = dalloc<double>(s.W);
+    afq::launch_ek(s, nullptr, s.W, d, nullptr, 0, c->stream);
+    launch_check("ek");
+    download(ek, d, s.W, c->stream);
+    (void)hipFree(d);
+  }
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_surrogate_features(afq_ctx* c, float* X) {
+  if (!c || !X) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  const int F = s.G + 2 * s.N;
+  float* d = dalloc<float>((int64_t)s.W * F);
+  afq::launch_features(s, nullptr, s.W, d, 0, c->stream);
+  launch_check("features");
+  hip_check(scopy(c, X, d, (size_t)s.W * F * 4, hipMemcpyDeviceToHost), "D2H");
+  (void)hipFree(d);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_set_cv(afq_ctx* c, int32_t subset_inv, double risk_frac, double fallback_rms, uint64_t seed) {
+  if (!c || subset_inv < 1) return c ? fail(c, AFQ_ERR_ARG, "subset_inv must be >= 1") : AFQ_ERR_ARG;
+  if (!(risk_frac >= 0.0 && risk_frac <= 1.0)) return fail(c, AFQ_ERR_ARG, "risk_frac must lie in [0, 1]");
+  c->cv_minv = subset_inv;
+  c->cv_risk = risk_frac;
+  c->cv_fallback = fallback_rms;
+  c->cv_rng.seed(seed);
+  return AFQ_OK;
+}
+
+int afq_cv_stats(afq_ctx* c, double* out6) {
+  if (!c || !out6) return AFQ_ERR_ARG;
+  for (int i = 0; i < 6; ++i) out6[i] = c->cv_stats[i];
   return AFQ_OK;
 }
 

@@ -368,4 +368,93 @@ struct MmaBF16 {
   }
 };
 
+// ---------------------------------------------------------------------------
+// BF16 engine with element loaders: la(r, k) and lb(k, c) return floats that
+// are rounded to bf16 when staged in LDS (the MLP's GEMMs, where operands come
+// from FP32 master weights / activations in several layouts).
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WAVES_M, int WAVES_N>
+struct MmaBF16E {
+  static constexpr int BK = 32;
+  static constexpr int LDK = BK + 8;
+  static constexpr int NT = 64 * WAVES_M * WAVES_N;
+  static constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  static constexpr int TM = WM / 16, TN = WN / 16;
+  static constexpr int A_PER_T = BM * BK / NT, B_PER_T = BK * BN / NT;
+  static constexpr int LDS_BYTES = (BM + BN) * LDK * 2;
+  static_assert((BM * BK) % NT == 0 && (BK * BN) % NT == 0, "tile not divisible by threads");
+  f32x4 acc[TM][TN];
+  int lane, wm, wn;
+  __device__ __forceinline__ MmaBF16E() {
+    lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    wm = wave / WAVES_N;
+    wn = wave % WAVES_N;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  template <class LA, class LB>
+  __device__ __forceinline__ void run(const LA& la, const LB& lb, int K, void* ldsv) {
+    u16* As = reinterpret_cast<u16*>(ldsv);
+    u16* Bs = As + BM * LDK;
+    const int tid = threadIdx.x;
+    u16 ra[A_PER_T], rb[B_PER_T];
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < A_PER_T; ++q) {
+        const int e = tid + q * NT, r = e / BK, kk = e % BK;
+        ra[q] = f2bf_bits(la(r, k0 + kk));
+      }
+#pragma unroll
+      for (int q = 0; q < B_PER_T; ++q) {
+        const int e = tid + q * NT, kk = e / BN, c = e % BN;
+        rb[q] = f2bf_bits(lb(k0 + kk, c));
+      }
+    };
+    const int nst = (K + BK - 1) / BK;
+    if (nst > 0) fetch(0);
+    for (int s = 0; s < nst; ++s) {
+#pragma unroll
+      for (int q = 0; q < A_PER_T; ++q) {
+        const int e = tid + q * NT, r = e / BK, kk = e % BK;
+        As[r * LDK + kk] = ra[q];
+      }
+#pragma unroll
+      for (int q = 0; q < B_PER_T; ++q) {
+        const int e = tid + q * NT, kk = e / BN, c = e % BN;
+        Bs[c * LDK + kk] = rb[q];
+      }
+      __syncthreads();
+      if (s + 1 < nst) fetch((s + 1) * BK);
+      const int koff = 8 * (lane >> 4);
+      bf16x8 a[TM], b[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        a[i] = *reinterpret_cast<const bf16x8*>(As + (wm * WM + i * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        b[j] = *reinterpret_cast<const bf16x8*>(Bs + (wn * WN + j * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      __syncthreads();
+    }
+  }
+  __device__ __forceinline__ int row(int i, int r) const { return wm * WM + i * 16 + 4 * (lane >> 4) + r; }
+  __device__ __forceinline__ int col(int j) const { return wn * WN + j * 16 + (lane & 15); }
+  template <class EP>
+  __device__ __forceinline__ void store(const EP& ep) const {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ep(row(i, r), col(j), acc[i][j][r]);
+  }
+};
+
 }  // namespace afq

@@ -4,6 +4,7 @@
 // E_L clip, and comb reconfiguration.  Single GPU; nranks virtual ranks model
 // an `mpirun -np nranks` run of the reference (rank streams, rank-0 comb,
 // rank-local running averages).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -12,12 +13,18 @@
 
 #include "../../include/vmc_afqmc.h"
 
+void afq_keep_error(const afq_ctx* c, int rc);
+
 namespace {
 
 struct CtxGuard {
   afq_ctx* c = nullptr;
+  int rc = AFQ_OK;
   ~CtxGuard() {
-    if (c) afq_destroy(c);
+    if (c) {
+      afq_keep_error(c, rc);
+      afq_destroy(c);
+    }
   }
 };
 
@@ -46,7 +53,8 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   // the reference's default cap uses the per-rank walker count (DQMC/MixedEstimator.cpp:452-454)
   cp.weight_cap = p->weight_cap > 0 ? p->weight_cap : std::max(100., nw / 10.);
   CtxGuard g;
-  int rc = afq_create(ham, left, &cp, &g.c);
+  int& rc = g.rc;
+  rc = afq_create(ham, left, &cp, &g.c);
   if (rc) return rc;
   afq_ctx* c = g.c;
   const size_t ms = (size_t)N * (na + nb);
@@ -116,6 +124,22 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   double propTime = 0., eneTime = 0., tStart = 0.;
   int64_t timed = 0;
   afq_stage_times(c, nullptr, 1);
+  const bool cv = p->energy_mode == AFQ_ENERGY_SURROGATE;
+  const int ntrain = cv ? std::max(1, p->surrogate_train_steps) : 0;
+  double cv_var = 0.0, cv_rms = 0.0;
+  int64_t cv_n = 0, nsamples = 0;
+  double tr_rmse = 0.0, va_rmse = 0.0;
+  if (cv) {
+    const int hidden = p->surrogate_hidden > 0 ? p->surrogate_hidden : 256;
+    const int64_t cap = (int64_t)ntrain * W;
+    if ((rc = afq_surrogate_init(c, hidden, (uint64_t)p->seed + 17, cap))) return rc;
+    if ((rc = afq_set_cv(c, p->subset_inv > 0 ? p->subset_inv : 16, p->risk_frac, p->fallback_rms,
+                         (uint64_t)p->seed + 29)))
+      return rc;
+  }
+  int64_t nmeasured = 0;
+  double est_var = 0.0;
+  int64_t est_n = 0;
   const int64_t last = (int64_t)nsweeps * nsteps;
   for (int64_t step = 1; step < last; ++step) {
     if (p->max_steps > 0 && step > p->max_steps) break;
@@ -151,13 +175,32 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     if (step % nsteps == 0) {
       const int block = (int)(step / nsteps);
       t0 = now_s();
-      if ((rc = afq_measure(c, p->energy_mode, delta, eloc.data(), nullptr))) return rc;
+      const bool exact_now = !cv || nmeasured < ntrain;
+      // training pairs come from the FP64 path whatever the production precision
+      if (cv && exact_now && (rc = afq_set_precision(c, AFQ_PREC_FP64, AFQ_PREC_FP64, AFQ_PREC_FP64))) return rc;
+      if ((rc = afq_measure(c, exact_now ? AFQ_ENERGY_EXACT : AFQ_ENERGY_SURROGATE, delta, eloc.data(), nullptr)))
+        return rc;
+      if (cv && exact_now && (rc = afq_set_precision(c, p->prec_fb, p->prec_vhs, p->prec_exchange))) return rc;
       eneTime += now_s() - t0;
+      ++nmeasured;
+      if (cv && exact_now) {
+        if ((rc = afq_surrogate_collect(c, &nsamples))) return rc;
+        if (nmeasured == ntrain) {
+          const int epochs = p->surrogate_epochs > 0 ? p->surrogate_epochs : 20;
+          if ((rc = afq_surrogate_train(c, epochs, 512, 1e-3, (uint64_t)p->seed + 31, &tr_rmse, &va_rmse))) return rc;
+        }
+      } else if (cv && step > p->warmup_steps) {
+        double st[6];
+        afq_cv_stats(c, st);
+        cv_var += st[3];
+        cv_rms += st[1];
+        ++cv_n;
+      }
       for (int r = 0; r < R; ++r)
         for (int wl = 0; wl < nw; ++wl) {
           const int w = r * nw + wl;
           if (weights[w] == 0.) { eloc[w] = 0.; continue; }
-          if (std::isnan(eloc[w]) || std::isinf(eloc[w])) return AFQ_ERR_NUMERIC;
+          if (std::isnan(eloc[w]) || std::isinf(eloc[w])) return rc = AFQ_ERR_NUMERIC;
           if (std::fabs(eloc[w] - averageEnergy[r]) > std::sqrt(2. / dt)) {
             ++nLarge;
             eloc[w] = eloc[w] > averageEnergy[r] ? averageEnergy[r] + std::sqrt(2. / dt)
@@ -168,6 +211,15 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
       for (int w = 0; w < W; ++w) we += eloc[w] * weights[w];
       totalWeights[block] = totalWeight;
       totalEnergies[block] = we / totalWeight;
+      if (step > p->warmup_steps) {  // variance of this block's weighted mean over the population
+        double v = 0.0;
+        for (int w = 0; w < W; ++w) {
+          const double d = eloc[w] - totalEnergies[block];
+          v += weights[w] * weights[w] * d * d;
+        }
+        est_var += v / (totalWeight * totalWeight);
+        ++est_n;
+      }
       if (step * dt < 10. || step < (int64_t)p->burn_iter * nsteps) {
         averageNumEql += totalEnergies[block] * totalWeights[block];
         averageDenomEql += totalWeights[block];
@@ -203,6 +255,15 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     res->timed_steps = timed;
     res->timed_wall_s = timed > 0 ? tEnd - tStart : 0.0;
     afq_stage_times(c, res->stage_ms, 1);
+    res->train_rmse = tr_rmse;
+    res->valid_rmse = va_rmse;
+    res->cv_var_sub = cv_n ? cv_var / cv_n : 0.0;
+    res->cv_rms = cv_n ? cv_rms / cv_n : 0.0;
+    double st[6] = {0, 0, 0, 0, 0, 0};
+    if (cv) afq_cv_stats(c, st);
+    res->cv_fallbacks = (int64_t)st[5];
+    res->train_samples = nsamples;
+    res->est_var = est_n ? est_var / est_n : 0.0;
   }
   return AFQ_OK;
 }

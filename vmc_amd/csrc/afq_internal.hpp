@@ -23,7 +23,8 @@ struct Slot {
 };
 
 // Per-kernel device time buckets (afq_stage_times order).
-enum Stage { ST_FB = 0, ST_FIELDS, ST_VHS, ST_PROP, ST_GREEN, ST_WEIGHT, ST_ORTHO, ST_ENERGY, ST_COMB, ST_COUNT };
+enum Stage { ST_FB = 0, ST_FIELDS, ST_VHS, ST_PROP, ST_GREEN, ST_WEIGHT, ST_ORTHO, ST_ENERGY, ST_COMB, ST_TAYLOR,
+             ST_COUNT };
 
 struct DevState {
   // sizes
@@ -73,13 +74,52 @@ struct DevState {
   double* cum = nullptr;     // W (comb)
   int* idx = nullptr;        // W (comb)
   unsigned long long* ncap = nullptr;
+
+  // mixed precision
+  int prec_exchange = AFQ_PREC_FP64;
+  int Np = 0;               // N padded to a multiple of 8 (bf16 rows)
+  int ctot = 0;             // theta columns per walker, 2 sum_s n_s
+  unsigned short* Rb_hi = nullptr; unsigned short* Rb_lo = nullptr;  // [slot][g][i][Np]
+  unsigned short* Tb_hi = nullptr; unsigned short* Tb_lo = nullptr;  // [w][ctot][Np]
 };
+
+// Exchange-energy surrogate: GELU MLP dims[0] -> dims[1] -> dims[2] -> dims[3] -> 1,
+// FP32 master weights w[l] (dims[l] x dims[l+1], row-major), Adam moments,
+// activation buffers for up to Bcap rows, and the training set (X: n x F, Y: n).
+struct Mlp {
+  int dims[5] = {0, 0, 0, 0, 0};
+  float* w[4] = {}; float* b[4] = {};
+  float* mw[4] = {}; float* vw[4] = {}; float* mb[4] = {}; float* vb[4] = {};
+  float* gw[4] = {}; float* gb[4] = {};
+  float* z[4] = {}; float* h[4] = {}; float* dh[4] = {};
+  float* mu = nullptr; float* sd = nullptr;
+  float t_mu = 0.f, t_sd = 1.f;
+  float* lossbuf = nullptr;
+  int Bcap = 0;
+  int t = 0;
+  float* X = nullptr; float* Y = nullptr;   // features, target E_K - E_J
+  int64_t n = 0, cap = 0;
+  bool ready = false;
+};
+void mlp_forward(const Mlp& m, const float* X, const int* idx, int B, hipStream_t st);
+void mlp_backward(const Mlp& m, const float* X, const int* idx, int B, hipStream_t st);
+float mlp_loss(const Mlp& m, const float* t, const int* idx, int B, hipStream_t st);
+void mlp_adam(Mlp& m, float lr, hipStream_t st);
+void launch_features(const DevState& s, const int* idx, int n, float* X, int64_t dst0, hipStream_t st);
+void launch_ek(const DevState& s, const int* idx, int n, double* ek, float* yout, int64_t dst0, hipStream_t st);
+void launch_predict(const DevState& s, const Mlp& m, const float* X, int n, double* S, hipStream_t st);
+void launch_ej(const DevState& s, double* ej, hipStream_t st);
+void launch_energy_cv(const DevState& s, const double* S, const double* ek, const float* pinv, double delta,
+                      hipStream_t st);
 
 // kernel launchers (afq_kernels.hip)
 void launch_force_bias(const DevState& s, hipStream_t st);
 void launch_fields(const DevState& s, const double* draws, uint64_t seed, uint64_t step, hipStream_t st);
 void launch_vhs(const DevState& s, hipStream_t st);
-void launch_propagate(const DevState& s, hipStream_t st);
+void launch_propagate(const DevState& s, hipStream_t st);  // E psi, Taylor terms, E acc
+void launch_expmul_pre(const DevState& s, hipStream_t st);
+void launch_taylor(const DevState& s, hipStream_t st);
+void launch_expmul_post(const DevState& s, hipStream_t st);
 void launch_green(const DevState& s, bool all_walkers, hipStream_t st);
 void launch_weight(const DevState& s, double eshift, bool update_weights, hipStream_t st);
 void launch_logO_from_logdet(const DevState& s, hipStream_t st);
@@ -98,6 +138,9 @@ void launch_debug_gemm_bf16(const double* A, const double* B, float* C, int M, i
 void launch_copy_calib(const double* a, double* b, int64_t n, hipStream_t st);
 void launch_fill(double* p, int64_t n, double v, hipStream_t st);
 void launch_zero_counter(unsigned long long* p, hipStream_t st);
+
+void launch_rsplit(const DevState& s, hipStream_t st);
+void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, hipStream_t st);
 
 int exchange_tile_g(int n);   // g-blocks per exchange tile for slot size n (0: unsupported)
 int exchange_tile_w(int n);   // walkers per exchange tile

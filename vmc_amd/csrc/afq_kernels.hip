@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_vmul(const double* P, const double* tsr
 }
 
 // psi <- E (sum_{k<=5} V^k / k!) E psi  for every live walker
-void launch_propagate(const DevState& s, hipStream_t st) {
+void launch_expmul_pre(const DevState& s, hipStream_t st) {
   const int64_t ws = 2 * s.ms;
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
@@ -287,6 +287,9 @@ void launch_propagate(const DevState& s, hipStream_t st) {
     hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.psi, s.t0, s.accb, s.weights, s.N,
                        n, s.W, ws, 2 * s.det_off[d], 0, gx, gy);
   }
+}
+void launch_taylor(const DevState& s, hipStream_t st) {
+  const int64_t ws = 2 * s.ms;
   const int n2a = 2 * s.det_n[0];
   const int n2b = s.ndets_prop > 1 ? 2 * s.det_n[1] : 0;
   const int64_t offb = 2 * s.det_off[1];
@@ -299,12 +302,20 @@ void launch_propagate(const DevState& s, hipStream_t st) {
                        s.T, n2a, n2b, offb, ws, sq, 1.0 / p, gx, gy, s.W);
     double* t = src; src = dst; dst = t;
   }
+}
+void launch_expmul_post(const DevState& s, hipStream_t st) {
+  const int64_t ws = 2 * s.ms;
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
     const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
     hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.accb, s.psi, nullptr, s.weights,
                        s.N, n, s.W, ws, 2 * s.det_off[d], 1, gx, gy);
   }
+}
+void launch_propagate(const DevState& s, hipStream_t st) {
+  launch_expmul_pre(s, st);
+  launch_taylor(s, st);
+  launch_expmul_post(s, st);
 }
 
 // ------------------------------------------------------------------ Green's function
@@ -728,6 +739,182 @@ void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st
                          s.Xp + 2 * (int64_t)z * s.W * s.ngt_cap, s.N, s.GE, nsel, s.W, s.Ktot, s.slot[z],
                          s.slot[z], s.ngt_cap, ngt, gy, 1);
     }
+  }
+}
+
+// ------------------------------------------------------------------ mixed-precision exchange
+// bf16 hi/lo copies, K (orbital) contiguous and padded to Np:
+//   Rb[c][slot][g][i][Np]          from R[g][koff_s + k*n_s + i]
+//   Tb[c][w][col][Np], col = cols_off_s + (j, re/im)   from theta_s[w][k][j][re/im]
+__global__ void k_rsplit(const double* R, u16* hi, u16* lo, int N, int Np, int G, int64_t Ktot, int64_t koff,
+                         int n, int64_t dst_off) {
+  const int64_t total = (int64_t)G * n * Np;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e % Np);
+    const int64_t gi = e / Np;
+    const int i = (int)(gi % n), g = (int)(gi / n);
+    const double x = k < N ? R[(int64_t)g * Ktot + koff + (int64_t)k * n + i] : 0.0;
+    u16 h, l;
+    split_bf16(x, h, l);
+    hi[dst_off + e] = h;
+    lo[dst_off + e] = l;
+  }
+}
+void launch_rsplit(const DevState& s, hipStream_t st) {
+  int64_t off = 0;
+  for (int q = 0; q < s.nslots; ++q) {
+    hipLaunchKernelGGL(k_rsplit, dim3(2048), dim3(256), 0, st, s.R, s.Rb_hi, s.Rb_lo, s.N, s.Np, s.G, s.Ktot,
+                       s.slot[q].koff, s.slot[q].n, off);
+    off += (int64_t)s.G * s.slot[q].n * s.Np;
+  }
+}
+
+// transpose + split theta of the selected walkers (one workgroup per walker and slot)
+__global__ __launch_bounds__(256) void k_theta_split(const double* theta, const int* sel, u16* hi, u16* lo, int N,
+                                                     int Np, int64_t Ktot, Slot s0, Slot s1, int ctot) {
+  __shared__ double tile[32 * 129];
+  const int w = sel ? sel[blockIdx.x] : blockIdx.x;
+  const Slot sl = blockIdx.y == 0 ? s0 : s1;
+  const int n2 = 2 * sl.n;
+  const int coff = blockIdx.y == 0 ? 0 : 2 * s0.n;
+  const double* th = theta + (int64_t)w * 2 * Ktot + 2 * sl.koff;
+  for (int r0 = 0; r0 < Np; r0 += 32) {
+    for (int e = threadIdx.x; e < 32 * n2; e += blockDim.x) {
+      const int r = e / n2, c = e - r * n2;
+      tile[r * 129 + c] = (r0 + r < N) ? th[(int64_t)(r0 + r) * n2 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * n2; e += blockDim.x) {
+      const int c = e / 32, r = e - c * 32;
+      if (r0 + r < Np) {
+        u16 h, l;
+        split_bf16(tile[r * 129 + c], h, l);
+        const int64_t a = ((int64_t)w * ctot + coff + c) * Np + r0 + r;
+        hi[a] = h;
+        lo[a] = l;
+      }
+    }
+    __syncthreads();
+  }
+}
+void launch_theta_split(const DevState& s, const int* sel, int nsel, hipStream_t st) {
+  hipLaunchKernelGGL(k_theta_split, dim3(nsel, s.nslots), dim3(256), 0, st, s.theta, sel, s.Tb_hi, s.Tb_lo, s.N,
+                     s.Np, s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ctot);
+}
+
+// Xp as k_exchange, f = R_g theta from bf16 hi/lo operands (NS = 2: BF16x3, NS = 1: BF16),
+// FP32 accumulation, the f_ij f_ji reduction in FP64.
+template <int NS>
+__global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u16* Rl, const u16* Th, const u16* Tl,
+                                                          const int* sel, double* Xp, int Np, int GE, int nw, int W,
+                                                          int G, Slot s0, Slot s1, int ctot, int ngt, int gridx,
+                                                          int gridy, int gridz) {
+  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS>;
+  constexpr int FT_BYTES = (XC_BN / 2) * XC_BN * 4;
+  constexpr int LDSB = Eng::LDS_BYTES > FT_BYTES ? Eng::LDS_BYTES : FT_BYTES;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[LDSB];
+  const TileId tl = tile_id(gridx, gridy, gridz, 8);
+  const int zs = gridz > 1 ? tl.z : 0;
+  const Slot sl = zs == 0 ? s0 : s1;
+  const int n = sl.n, gt = XC_BM / n, wt = XC_BN / (2 * n);
+  const int g0 = tl.x * gt, w0 = tl.y * wt;
+  const int coff = (zs == 0 && sl.koff == s0.koff) ? 0 : 2 * s0.n;
+  const int64_t roff = (zs == 0 && sl.koff == s0.koff) ? 0 : (int64_t)G * s0.n * Np;
+  Eng e;
+  e.run(
+      [&](int c, int r) -> const u16* {
+        const int gl = r / n, i = r - gl * n, g = g0 + gl;
+        if (gl >= gt || g >= GE) return nullptr;
+        return (c ? Rl : Rh) + roff + ((int64_t)g * n + i) * Np;
+      },
+      [&](int c, int q) -> const u16* {
+        const int wl = q / (2 * n), cc = q - wl * 2 * n;
+        if (wl >= wt || w0 + wl >= nw) return nullptr;
+        const int w = sel ? sel[w0 + wl] : w0 + wl;
+        return (c ? Tl : Th) + ((int64_t)w * ctot + coff + cc) * Np;
+      },
+      Np, lds);
+  float* F = reinterpret_cast<float*>(lds);
+  const int tid = threadIdx.x, nr = wt * n;
+  double accr = 0.0, acci = 0.0;
+  for (int gl = 0; gl < gt; ++gl) {
+    if (g0 + gl >= GE) break;
+    e.store([&](int r, int c, float v) {
+      const int rr = r - gl * n;
+      if (rr >= 0 && rr < n) F[rr * XC_BN + c] = v;
+    });
+    __syncthreads();
+    if (tid < nr) {
+      const int wl = tid / n, i = tid - wl * n;
+      const float* Fb = F + wl * 2 * n;
+      double sr = 0.0, si = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double ar = Fb[i * XC_BN + 2 * j], ai = Fb[i * XC_BN + 2 * j + 1];
+        const double br = Fb[j * XC_BN + 2 * i], bi = Fb[j * XC_BN + 2 * i + 1];
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
+      }
+      accr += sr;
+      acci += si;
+    }
+    __syncthreads();
+  }
+  double* red = reinterpret_cast<double*>(lds);
+  if (tid < nr) {
+    red[2 * tid] = accr;
+    red[2 * tid + 1] = acci;
+  }
+  __syncthreads();
+  if (tid < wt && w0 + tid < nw) {
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < n; ++i) {
+      sr += red[2 * (tid * n + i)];
+      si += red[2 * (tid * n + i) + 1];
+    }
+    const int w = sel ? sel[w0 + tid] : w0 + tid;
+    double* o = Xp + 2 * (((int64_t)zs * W + w) * ngt + tl.x);
+    o[0] = sr;
+    o[1] = si;
+  }
+}
+
+void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, hipStream_t st) {
+  launch_theta_split(s, sel, nsel, st);
+  auto go = [&](int z0, int nz, double* xp) {
+    const Slot& a = s.slot[z0];
+    const int gt = exchange_tile_g(a.n), wt = exchange_tile_w(a.n);
+    const int ngt = (int)cdivu(s.GE, gt), gy = (int)cdivu(nsel, wt);
+    const Slot s1 = nz > 1 ? s.slot[1] : a;
+    if (prec == AFQ_PREC_BF16X3)
+      hipLaunchKernelGGL(k_exchange_bf16<2>, dim3(ngt * gy * nz), dim3(256), 0, st, s.Rb_hi, s.Rb_lo, s.Tb_hi,
+                         s.Tb_lo, sel, xp, s.Np, s.GE, nsel, s.W, s.G, a, s1, s.ctot, s.ngt_cap, ngt, gy, nz);
+    else
+      hipLaunchKernelGGL(k_exchange_bf16<1>, dim3(ngt * gy * nz), dim3(256), 0, st, s.Rb_hi, s.Rb_lo, s.Tb_hi,
+                         s.Tb_lo, sel, xp, s.Np, s.GE, nsel, s.W, s.G, a, s1, s.ctot, s.ngt_cap, ngt, gy, nz);
+  };
+  if (s.nslots == 1 || s.slot[1].n == s.slot[0].n) go(0, s.nslots, s.Xp);
+  else {
+    go(0, 1, s.Xp);
+    // the second slot alone: present it as s0 with its own offsets
+    DevState t = s;
+    (void)t;
+    const Slot& b = s.slot[1];
+    const int gt = exchange_tile_g(b.n), wt = exchange_tile_w(b.n);
+    const int ngt = (int)cdivu(s.GE, gt), gy = (int)cdivu(nsel, wt);
+    // pass pointers advanced to slot 1's R rows / theta columns; koff sentinel makes coff/roff zero
+    Slot bb = b;
+    bb.koff = s.slot[0].koff;
+    const u16* rh = s.Rb_hi + (int64_t)s.G * s.slot[0].n * s.Np;
+    const u16* rl = s.Rb_lo + (int64_t)s.G * s.slot[0].n * s.Np;
+    const u16* th = s.Tb_hi + (int64_t)2 * s.slot[0].n * s.Np;
+    const u16* tlo = s.Tb_lo + (int64_t)2 * s.slot[0].n * s.Np;
+    double* xp = s.Xp + 2 * (int64_t)s.W * s.ngt_cap;
+    if (prec == AFQ_PREC_BF16X3)
+      hipLaunchKernelGGL(k_exchange_bf16<2>, dim3(ngt * gy), dim3(256), 0, st, rh, rl, th, tlo, sel, xp, s.Np, s.GE,
+                         nsel, s.W, s.G, bb, bb, s.ctot, s.ngt_cap, ngt, gy, 1);
+    else
+      hipLaunchKernelGGL(k_exchange_bf16<1>, dim3(ngt * gy), dim3(256), 0, st, rh, rl, th, tlo, sel, xp, s.Np, s.GE,
+                         nsel, s.W, s.G, bb, bb, s.ctot, s.ngt_cap, ngt, gy, 1);
   }
 }
 

@@ -6,9 +6,13 @@ Generator (same for every workload):
         ERI (ij|kl) = sum_g L_g,ij L_g,kl is positive semidefinite by construction
   hcore_mod = h - 1/2 sum_g L_g L_g   (scripts/QMCUtils.py:644-645)
   ecore = 0
-Trial: lowest-n eigenvectors of the mean-field Fock matrix F = h + J - K built from
-the core-guess density (RHF), or a short UHF iteration started from that RHF with
-a small alpha/beta HOMO-LUMO rotation (UHF).
+Trial: the lowest-n eigenvectors of the self-consistent mean-field Fock matrix
+F = h + J - K (RHF: F = h + 2J[D] - K[D]; UHF: F_s = h + J[D_a + D_b] - K[D_s]),
+iterated from the core guess with Pulay DIIS on the commutator F D - D F until it
+is below 1e-6 (UHF starts from opposite alpha/beta HOMO-LUMO rotations so that
+spin symmetry can break).  A single Fock build from the core guess leaves these
+strongly interacting Hamiltonians tens to hundreds of Ha above the mean-field
+solution, which makes the walk and its local energies needlessly wild.
 
 Workloads (names used by tests and bench.py):
   c0  N=10,  5a+5b,   G=40,   RHF trial, 128 walkers   (reference-CPU-runnable case)
@@ -41,8 +45,47 @@ def _jk(L, C):
     return J, K
 
 
-def make_system(norb, nalpha, nbeta, nchol, seed=DEFAULT_SEED, trial="rhf",
-                uhf_iters=8, uhf_kick=0.15):
+def _diis_scf(h, L, Cs, ns, spin_resolved, max_iter=120, tol=1e-6, nkeep=8):
+    """Self-consistent field with Pulay DIIS.  Cs: starting orbitals per spin."""
+    nspin = len(Cs)
+    Fs_hist, err_hist = [], []
+    for it in range(max_iter):
+        JK = [_jk(L, C) for C in Cs]
+        if spin_resolved:
+            Jt = JK[0][0] + JK[1][0]
+            Fs = [h + Jt - JK[s][1] for s in range(2)]
+        else:
+            J, K = JK[0]
+            Fs = [h + 2.0 * J - K]
+        Ds = [C @ C.T for C in Cs]
+        errs = [F @ D - D @ F for F, D in zip(Fs, Ds)]
+        err = max(np.abs(e).max() for e in errs)
+        if err < tol:
+            break
+        Fs_hist.append([F.copy() for F in Fs])
+        err_hist.append(np.concatenate([e.ravel() for e in errs]))
+        if len(Fs_hist) > nkeep:
+            Fs_hist.pop(0)
+            err_hist.pop(0)
+        m = len(Fs_hist)
+        if m > 1:
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for i in range(m):
+                for j in range(m):
+                    B[i, j] = err_hist[i] @ err_hist[j]
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            try:
+                c = np.linalg.solve(B, rhs)[:m]
+                Fs = [sum(c[i] * Fs_hist[i][s] for i in range(m)) for s in range(nspin)]
+            except np.linalg.LinAlgError:
+                pass
+        Cs = [_eigh_lowest(F, n) for F, n in zip(Fs, ns)]
+    return Cs
+
+
+def make_system(norb, nalpha, nbeta, nchol, seed=DEFAULT_SEED, trial="rhf", uhf_kick=0.3):
     rng = np.random.default_rng(seed)
     N = norb
     A = rng.standard_normal((N, N))
@@ -57,30 +100,18 @@ def make_system(norb, nalpha, nbeta, nchol, seed=DEFAULT_SEED, trial="rhf",
         L[g] = M + np.triu(M, 1).T
     M = L.reshape(nchol * N, N)          # sum_g L_g L_g = M^T M for symmetric L_g
     h_mod = h - 0.5 * (M.T @ M)
-    # RHF from the core guess
-    C0 = _eigh_lowest(h, nalpha)
-    J, K = _jk(L, C0)
-    F = h + 2.0 * J - K
-    Ca = _eigh_lowest(F, nalpha)
-    Cb = _eigh_lowest(F, nbeta)
+    C0 = _eigh_lowest(h, max(nalpha, nbeta))  # core guess
     if trial == "uhf":
-        # break spin symmetry with opposite HOMO-LUMO rotations, then iterate
-        w, v = np.linalg.eigh(F)
+        w, v = np.linalg.eigh(h)
         def rot(n, sgn):
             C = v[:, :n].copy()
             if n < N:
                 c, s = np.cos(uhf_kick), np.sin(sgn * uhf_kick)
-                homo, lumo = v[:, n - 1].copy(), v[:, n].copy()
-                C[:, n - 1] = c * homo + s * lumo
+                C[:, n - 1] = c * v[:, n - 1] + s * v[:, n]
             return C
-        Ca, Cb = rot(nalpha, 1.0), rot(nbeta, -1.0)
-        for _ in range(uhf_iters):
-            Ja, Ka = _jk(L, Ca)
-            Jb, Kb = _jk(L, Cb)
-            Fa = h + Ja + Jb - Ka
-            Fb = h + Ja + Jb - Kb
-            Ca, Cb = _eigh_lowest(Fa, nalpha), _eigh_lowest(Fb, nbeta)
+        Ca, Cb = _diis_scf(h, L, [rot(nalpha, 1.0), rot(nbeta, -1.0)], [nalpha, nbeta], True)
         return HamArrays(N, nalpha, nbeta, 0.0, h, h_mod, L, Ca, Cb, trial_rhf=False)
+    (Ca,) = _diis_scf(h, L, [C0[:, :nalpha]], [nalpha], False)
     return HamArrays(N, nalpha, nbeta, 0.0, h, h_mod, L, Ca, Ca[:, :nbeta], trial_rhf=True)
 
 
