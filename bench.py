@@ -4,8 +4,18 @@
 A step is one pass of the walker step over the whole population: force bias,
 field draws (device counter-based RNG), VHS, Taylor propagation, Green's
 function / overlap, phaseless weights, population control, QR every 5 steps,
-and the local energy of every walker (exact, or surrogate + exact subset in the
-control-variate path).  Inputs are resident in HBM before the timed region.
+and the local energy of every walker.  Inputs are resident in HBM before the
+timed region.  Paths:
+
+  fp64               exact FP64 local energy of every walker;
+  mixed_exchange_cv  control variate: the exchange in one BF16 MFMA pass on every
+                     walker, exact FP64 on a random 1/subset_inv of them
+                     (Horvitz-Thompson correction, unbiased); the headline value;
+  mlp_cv             control variate with the learned MLP surrogate, trained on
+                     the GPU from the first exact steps.
+
+Both control-variate paths also measure every block exactly in FP64 outside the
+timings ("shadow"), which gives the energy shift on identical walkers.
 
 Workload (default): N=300 orbitals, 60+60 electrons, 1200 Cholesky vectors,
 UHF trial, 4096 walkers, dt=0.005 -- synthetic Hamiltonian (vmc_amd.synthetic).
@@ -57,7 +67,7 @@ def flops_per_walker_step(spec, ortho=5, energy=True):
     return f
 
 
-STAGES = ["force_bias", "fields", "vhs", "onebody", "green", "weight", "ortho", "exchange", "comb", "taylor"]
+from vmc_amd.afqmc import STAGES  # noqa: E402
 PEAKS = {"fp64": 78.6, "bf16": 2500.0}
 
 
@@ -79,27 +89,39 @@ def run_path(ham, spec, total_steps, warmup, **kw):
                 est_var=float(res.est_var), err_per_sqrt_s=float(np.sqrt(res.est_var * ms * 1e-3)),
                 train_rmse=float(res.train_rmse), valid_rmse=float(res.valid_rmse),
                 cv_var_sub=float(res.cv_var_sub), cv_rms=float(res.cv_rms),
-                cv_fallbacks=int(res.cv_fallbacks), train_samples=int(res.train_samples))
+                cv_fallbacks=int(res.cv_fallbacks), train_samples=int(res.train_samples),
+                shadow_n=int(res.shadow_n), shadow_shift=float(res.shadow_shift),
+                shadow_var=float(res.shadow_var), shadow_est_var=float(res.shadow_est_var),
+                shadow_wall_s=float(res.shadow_wall_s), live_frac=float(res.live_frac))
 
 
 def stage_rooflines(spec, path, n_exact):
     """Algorithmic FLOP/s of each stage of one step (per launch for the kernels that
-    launch several times) against the MFMA peak of the precision it runs in."""
+    launch several times) against the MFMA peak of the precision it runs in.
+    n_exact: walkers whose exchange runs in FP64 in one step."""
     N, G, W = spec["norb"], spec["nchol"], spec["nwalk"]
     T = N * (N + 1) / 2
     nsp = [spec["nalpha"]] if spec["walker_rhf"] else [spec["nalpha"], spec["nbeta"]]
     st = path["stage_ms_per_step"]
     out = {}
-    def add(name, kernel, flops, launches, prec):
-        ms = st[name] / launches
+    def add(name, kernel, flops, launches, prec, **extra):
+        ms = st.get(name, 0.0) / launches
         if ms <= 0:
             return
         ach = flops / launches / (ms * 1e-3) / 1e12
         out[name] = dict(kernel=kernel, launch_ms=ms, flops_per_launch=flops / launches,
                          achieved=ach, peak=PEAKS[prec], frac=ach / PEAKS[prec], unit="TFLOP/s",
-                         bound="mfma", precision=prec)
-    add("exchange", "k_exchange / k_exchange_bf16", exchange_flops(spec, n_exact), 1,
-        "fp64" if path["exchange_prec"] == "fp64" else "bf16")
+                         bound="mfma", precision=prec, **extra)
+    if path["exchange_prec"] == "fp64":
+        add("exchange", "k_exchange (FP64 MFMA)", exchange_flops(spec, n_exact), 1, "fp64",
+            walkers=n_exact)
+    else:
+        add("exchange", "k_exchange_bf16 (3-pass split BF16 MFMA)", exchange_flops(spec, n_exact), 1, "bf16",
+            walkers=n_exact, passes=3)
+    if path.get("surrogate_prec"):
+        passes = 3 if path["surrogate_prec"] == "bf16x3" else 1
+        add("exchange_lp", "k_exchange_bf16 (%d-pass BF16 MFMA, surrogate level)" % passes,
+            exchange_flops(spec, W), 1, "bf16", walkers=W, passes=passes)
     add("taylor", "k_vmul (5 complex V t products, FP64 MFMA)", W * sum(40.0 * N * N * n for n in nsp), 5, "fp64")
     add("onebody", "k_expmul (E psi, FP64 MFMA)", W * sum(8.0 * N * N * n for n in nsp), 2 * len(nsp), "fp64")
     add("force_bias", "k_force_bias (FP64 MFMA)", W * sum(4.0 * n * N * G for n in nsp), 1, "fp64")
@@ -107,6 +129,30 @@ def stage_rooflines(spec, path, n_exact):
     add("green", "k_green (FP64 MFMA + Gauss-Jordan)", W * sum(4 * n * n * N + 16 * n ** 3 + 8 * N * n * n
                                                               for n in nsp), 1, "fp64")
     return out
+
+
+def cv_summary(mx, fp64):
+    """Energy shift and statistical efficiency of a control-variate path.
+
+    On identical walkers (a shadow FP64 measurement of every surrogate block,
+    outside the timings): shift = mean(E_cv - E_fp64) per block with its standard
+    error (the control-variate noise is independent from block to block).  The
+    error per sqrt(wall-second) ratio charges that noise on top of the FP64
+    estimator's own per-block variance with no credit for autocorrelation of the
+    latter, so it is an upper bound."""
+    n = mx["shadow_n"]
+    sem = float(np.sqrt(mx["shadow_var"] / n)) if n > 1 else None
+    ratio = None
+    if mx["shadow_est_var"] > 0:
+        ratio = float(np.sqrt((1.0 + mx["shadow_var"] / mx["shadow_est_var"])
+                              * mx["ms_per_step"] / fp64["ms_per_step"]))
+    return dict(energy_shift_mha=1e3 * mx["shadow_shift"],
+                energy_shift_sem_mha=None if sem is None else 1e3 * sem,
+                energy_shift_blocks=n,
+                cv_noise_per_block_mha=1e3 * float(np.sqrt(mx["shadow_var"])),
+                fp64_error_per_block_mha=1e3 * float(np.sqrt(mx["shadow_est_var"])),
+                speedup_vs_fp64=mx["walker_steps_per_s"] / fp64["walker_steps_per_s"],
+                err_per_sqrt_s_ratio=ratio)
 
 
 def cpu_baseline(ham, spec, threads):
@@ -136,8 +182,10 @@ def main():
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--subset-inv", type=int, default=16)
     ap.add_argument("--risk-frac", type=float, default=0.05,
-                    help="fraction of live walkers (largest |E_J - median|) always measured exactly")
+                    help="learned-surrogate path: fraction of live walkers (largest |E_J - median|) "
+                         "always measured exactly")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the learned-surrogate path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
@@ -148,7 +196,7 @@ def main():
               "--gpus %d is not supported" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
-    from vmc_amd.abi import AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16X3
+    from vmc_amd.abi import AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16, AFQ_PREC_FP64, AFQ_SURR_EXCHANGE, AFQ_SURR_MLP
     from vmc_amd.synthetic import workload
     t0 = time.time()
     ham, spec = workload(a.workload)
@@ -158,55 +206,58 @@ def main():
         from vmc_amd.afqmc import debug_copy
         debug_copy(1 << 30)
     T = a.train_steps or max(2, (48 * 1024 + W - 1) // W)
-    if a.fp64_only:
-        T = 0
-    total = T + a.warmup + a.steps
-    fp64 = run_path(ham, spec, total, T + a.warmup)
-    fp64["exchange_prec"] = "fp64"
-    print("[bench] fp64: %.1f walker-steps/s, %.1f ms/step" % (fp64["walker_steps_per_s"], fp64["ms_per_step"]),
-          file=sys.stderr, flush=True)
-    paths = {"fp64": fp64}
-    best = fp64
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
+    paths = {}
+    fp64 = run_path(ham, spec, a.warmup + a.steps, a.warmup)
+    fp64.update(exchange_prec="fp64", n_exact=W)
+    paths["fp64"] = fp64
+    log("[bench] fp64: %.1f walker-steps/s, %.1f ms/step, live %.3f"
+        % (fp64["walker_steps_per_s"], fp64["ms_per_step"], fp64["live_frac"]))
+    best_name = "fp64"
     if not a.fp64_only:
-        mix = run_path(ham, spec, total, T + a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
-                       prec_exchange=AFQ_PREC_BF16X3, surrogate_train_steps=T, surrogate_hidden=256,
-                       surrogate_epochs=a.epochs, subset_inv=a.subset_inv, risk_frac=a.risk_frac)
-        mix["exchange_prec"] = "bf16x3"
-        print("[bench] mixed+surrogate: %.1f walker-steps/s, %.1f ms/step (train rmse %.3g, valid %.3g Ha)"
-              % (mix["walker_steps_per_s"], mix["ms_per_step"], mix["train_rmse"], mix["valid_rmse"]),
-              file=sys.stderr, flush=True)
-        paths["mixed_surrogate"] = mix
-        best = mix
+        # surrogate = the exchange in one BF16 pass on every walker; exact FP64 on a
+        # random 1/subset_inv of the walkers (Horvitz-Thompson correction)
+        xcv = run_path(ham, spec, a.warmup + a.steps, a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
+                       prec_exchange=AFQ_PREC_FP64, surrogate_kind=AFQ_SURR_EXCHANGE,
+                       surrogate_prec=AFQ_PREC_BF16, subset_inv=a.subset_inv, risk_frac=0.0, shadow_exact=True)
+        nx = (W + a.subset_inv - 1) // a.subset_inv
+        xcv.update(exchange_prec="fp64", surrogate_prec="bf16", n_exact=nx)
+        xcv.update(cv_summary(xcv, fp64))
+        paths["mixed_exchange_cv"] = xcv
+        log("[bench] mixed (bf16 exchange level + fp64 1/%d subset): %.1f walker-steps/s, %.1f ms/step, "
+            "speedup %.2fx, shift %.4f +- %s mHa, err/sqrt(s) ratio %s, live %.3f"
+            % (a.subset_inv, xcv["walker_steps_per_s"], xcv["ms_per_step"], xcv["speedup_vs_fp64"],
+               xcv["energy_shift_mha"], xcv["energy_shift_sem_mha"], xcv["err_per_sqrt_s_ratio"],
+               xcv["live_frac"]))
+        best_name = "mixed_exchange_cv"
+        if not a.no_mlp:
+            mlp = run_path(ham, spec, T + a.warmup + a.steps, T + a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
+                           prec_exchange=AFQ_PREC_FP64, surrogate_kind=AFQ_SURR_MLP, surrogate_train_steps=T,
+                           surrogate_hidden=256, surrogate_epochs=a.epochs, subset_inv=a.subset_inv,
+                           risk_frac=a.risk_frac, shadow_exact=True)
+            nrisk = int(np.ceil(a.risk_frac * W))
+            mlp.update(exchange_prec="fp64", n_exact=nrisk + (W - nrisk + a.subset_inv - 1) // a.subset_inv)
+            mlp.update(cv_summary(mlp, fp64))
+            paths["mlp_cv"] = mlp
+            log("[bench] mlp surrogate (train rmse %.3g, valid %.3g Ha): %.1f walker-steps/s, %.1f ms/step, "
+                "speedup %.2fx, shift %.4f +- %s mHa, err/sqrt(s) ratio %s, live %.3f"
+                % (mlp["train_rmse"], mlp["valid_rmse"], mlp["walker_steps_per_s"], mlp["ms_per_step"],
+                   mlp["speedup_vs_fp64"], mlp["energy_shift_mha"], mlp["energy_shift_sem_mha"],
+                   mlp["err_per_sqrt_s_ratio"], mlp["live_frac"]))
     for name, pth in paths.items():
-        nrisk = int(np.ceil(a.risk_frac * W))
-        n_exact = W if name == "fp64" else nrisk + (W - nrisk + a.subset_inv - 1) // a.subset_inv
-        pth["rooflines"] = stage_rooflines(spec, pth, n_exact)
+        pth["rooflines"] = stage_rooflines(spec, pth, pth["n_exact"])
+    best = paths[best_name]
     # the dominant kernel: largest share of the step
     shares = {k: best["stage_ms_per_step"][k] for k in best["rooflines"]}
     dom_name = max(shares, key=shares.get)
     dom = dict(best["rooflines"][dom_name])
     dom["stage"] = dom_name
     dom["traffic"] = None
-    result_energy = {}
-    if "mixed_surrogate" in paths:
-        e64, emx = fp64["energies"], paths["mixed_surrogate"]["energies"]
-        d = (emx - e64)[T + 1:]
-        result_energy = dict(
-            energy_shift_mha=1e3 * float(np.mean(d)), energy_shift_sem_mha=1e3 * float(np.std(d, ddof=1) / np.sqrt(len(d)))
-            if len(d) > 1 else None, max_abs_step_shift_mha=1e3 * float(np.max(np.abs(d))), compared_steps=len(d),
-            speedup_vs_fp64=paths["mixed_surrogate"]["walker_steps_per_s"] / fp64["walker_steps_per_s"],
-            err_per_sqrt_s_ratio=paths["mixed_surrogate"]["err_per_sqrt_s"] / fp64["err_per_sqrt_s"]
-            if fp64["err_per_sqrt_s"] > 0 else None)
-        print("[bench] speedup %.2fx, energy shift %.4f mHa (sem %s), error/sqrt(s) ratio %s"
-              % (result_energy["speedup_vs_fp64"], result_energy["energy_shift_mha"],
-                 result_energy["energy_shift_sem_mha"], result_energy["err_per_sqrt_s_ratio"]),
-              file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(ham, spec, threads)
-        print("[bench] cpu baseline: %.2f walker-steps/s on %d threads" % (cpu["value"], threads),
-              file=sys.stderr, flush=True)
+        log("[bench] cpu baseline: %.2f walker-steps/s on %d threads" % (cpu["value"], threads))
     wstep = flops_per_walker_step(spec)
     for pth in paths.values():
         pth.pop("energies", None)
@@ -218,24 +269,26 @@ def main():
         "unit": "walker-steps/s",
         "n_gpus": 1,
         "steps": best["steps"],
-        "warmup": T + a.warmup,
+        "warmup": a.warmup,
         "ms_per_step": best["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+bf16x3" if "mixed_surrogate" in paths else "f64",
+        "dtype": "f64" if best_name == "fp64" else "f64+bf16",
         "data": "synthetic",
         "config": {"workload": "N=%d %da+%db G=%d %s trial, %d walkers, dt=%g, E_L every step, QR every %d"
                    % (spec["norb"], spec["nalpha"], spec["nbeta"], spec["nchol"], spec["trial"].upper(),
                       W, spec["dt"], spec["ortho"]),
-                   "name": a.workload, "path": "mixed_surrogate" if "mixed_surrogate" in paths else "fp64",
-                   "subset_inv": a.subset_inv, "risk_frac": a.risk_frac, "surrogate_train_steps": T},
+                   "name": a.workload, "path": best_name, "subset_inv": a.subset_inv},
         "roofline": dom,
         "cpu_baseline": cpu,
         "paths": paths,
         "flops_per_walker_step": wstep,
     }
-    out.update(result_energy)
+    if best_name != "fp64":
+        for k in ("energy_shift_mha", "energy_shift_sem_mha", "energy_shift_blocks", "speedup_vs_fp64",
+                  "err_per_sqrt_s_ratio"):
+            out[k] = best[k]
     print(json.dumps(out))
 
 

@@ -160,10 +160,13 @@ int afq_propagate(afq_ctx* ctx, const double* draws, uint64_t seed, uint64_t ste
  * R_kk = -sign(Re x_0) |x|), trialOverlap /= prod diag R. */
 int afq_orthogonalize(afq_ctx* ctx);
 
-/* Local energies of every walker with weight != 0 (others 0).
+/* Local energies of every walker with weight != 0; dead walkers get eloc 0 and
+ * E*O = 0 (their exchange is not computed, as the reference computes no local
+ * energy for them, DQMC/MixedEstimator.cpp:531).
  * mode AFQ_ENERGY_EXACT: eloc[w] = Re(E*O/O) + delta.  ham_ovlp (nullable):
- * per walker {E*O, O} as two complex numbers (hamAndOverlap).  No clipping here;
- * see afq_clip_energies. */
+ * per walker {E*O, O} as two complex numbers (hamAndOverlap).  The exact mode does
+ * not clip (the caller clips, DQMC/MixedEstimator.cpp:525-530); the surrogate mode
+ * clips inside its estimate when afq_set_energy_clip set a clip. */
 int afq_measure(afq_ctx* ctx, int32_t mode, double delta, double* eloc, double* ham_ovlp);
 
 /* E_L of arbitrary determinants against the left trial (init-time helper used
@@ -185,9 +188,10 @@ int afq_comb(afq_ctx* ctx, double zeta, double* total_weight);
 
 /* Per-stage device time (ms) accumulated since the last reset, AFQ_NSTAGES
  * entries in the order force_bias, fields, vhs, one-body (E psi), green, weight,
- * ortho, exchange, comb, taylor (the five V t products). */
-#define AFQ_NSTAGES 10
-int afq_stage_times(afq_ctx* ctx, double* ms10, int32_t reset);
+ * ortho, exchange, comb, taylor (the five V t products), exchange_lp (the
+ * reduced-precision exchange of the AFQ_SURR_EXCHANGE surrogate). */
+#define AFQ_NSTAGES 11
+int afq_stage_times(afq_ctx* ctx, double* ms, int32_t reset);
 
 /* ---------------------------------------------------------------- surrogate
  * Exchange-energy surrogate (no counterpart in the reference): a GELU MLP
@@ -207,6 +211,10 @@ int afq_surrogate_init(afq_ctx* ctx, int32_t hidden, uint64_t seed, int64_t capa
 int afq_surrogate_collect(afq_ctx* ctx, int64_t* total);
 /* append n external (features X[n x F], target Y[n]) pairs to the training set */
 int afq_surrogate_load(afq_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t* total);
+/* Adam on the first nine tenths of the samples (collection order); the last tenth
+ * validates.  The weights of the epoch with the lowest validation RMSE are kept and
+ * the output is calibrated on the validation set to a + b * net, b = max(0,
+ * cov(y, net) / var(net)), so a network that predicts nothing becomes a constant. */
 int afq_surrogate_train(afq_ctx* ctx, int32_t epochs, int32_t batch, double lr, uint64_t seed,
                         double* train_rmse, double* valid_rmse);
 int afq_surrogate_predict(afq_ctx* ctx, double* pred);
@@ -221,6 +229,20 @@ int afq_set_cv(afq_ctx* ctx, int32_t subset_inv, double risk_frac, double fallba
 int afq_energy_parts(afq_ctx* ctx, double* e1, double* ek);
 /* surrogate input features of every walker (nwalk x (nchol + 2 norb), float) */
 int afq_surrogate_features(afq_ctx* ctx, float* X);
+/* Surrogate of the control variate (AFQ_ENERGY_SURROGATE):
+ *   AFQ_SURR_MLP       the trained network (default);
+ *   AFQ_SURR_EXCHANGE  the exchange contraction itself in reduced precision `prec`
+ *                      (AFQ_PREC_BF16 or AFQ_PREC_BF16X3) on every walker; the exact
+ *                      set is then evaluated in the context's exchange precision. */
+#define AFQ_SURR_MLP 0
+#define AFQ_SURR_EXCHANGE 1
+int afq_set_surrogate_kind(afq_ctx* ctx, int32_t kind, int32_t prec);
+/* Clip applied inside the control-variate estimate: the surrogate-level and the
+ * exact energy of walker w are both clipped to center[w] -+ halfwidth before the
+ * Horvitz-Thompson correction (DQMC/MixedEstimator.cpp:525-530 clips every local
+ * energy to the running average -+ sqrt(2/dt) on the host; clipping the corrected
+ * estimate instead would bias it).  center == NULL or halfwidth <= 0: no clip. */
+int afq_set_energy_clip(afq_ctx* ctx, const double* center, double halfwidth);
 /* {exact-set size, residual RMS of its random part (Ha), fallback used, subset-sampling
  *  variance estimate of the weighted mean energy (Ha^2), mean residual, cumulative
  *  fallbacks} of the last surrogate measurement */
@@ -258,13 +280,19 @@ typedef struct afq_run_params {
   int32_t device;
   /* surrogate control variate (energy_mode == AFQ_ENERGY_SURROGATE): the first
    * surrogate_train_steps steps measure exactly (FP64) and collect (features,
-   * E_K) pairs, then the MLP is trained and the control variate takes over */
+   * E_K - E_J) pairs, then the MLP is trained and the control variate takes over
+   * (AFQ_SURR_MLP; the AFQ_SURR_EXCHANGE surrogate needs no training) */
   int32_t surrogate_train_steps;
   int32_t surrogate_hidden;
   int32_t surrogate_epochs;
   int32_t subset_inv;    /* exact E_L on a 1/subset_inv stratified subset     */
   double fallback_rms;   /* exact fallback when the subset residual RMS exceeds this (0: never) */
   double risk_frac;      /* fraction of live walkers always measured exactly  */
+  int32_t shadow_exact;  /* 1: after each surrogate measurement also measure every walker
+                          * exactly in FP64 (excluded from the timings) and report the
+                          * difference of the two block energies on identical walkers */
+  int32_t surrogate_kind; /* AFQ_SURR_MLP (trained first) | AFQ_SURR_EXCHANGE (no training) */
+  int32_t surrogate_prec; /* precision of the AFQ_SURR_EXCHANGE level                      */
 } afq_run_params;
 
 typedef struct afq_run_result {
@@ -275,13 +303,19 @@ typedef struct afq_run_result {
   double prop_time, ene_time;  /* host wall seconds, as DQMC/MixedEstimator.cpp:664-676 */
   int64_t timed_steps;
   double timed_wall_s;         /* wall seconds of the steps after warmup_steps     */
-  double stage_ms[10];         /* device time per stage (afq_stage_times order)     */
+  double stage_ms[AFQ_NSTAGES]; /* device time per stage (afq_stage_times order)    */
   double train_rmse, valid_rmse; /* surrogate fit (Ha)                             */
   double cv_var_sub;           /* mean subset-sampling variance of the block energy */
   double cv_rms;               /* mean residual RMS of the exact subset (Ha)        */
   int64_t cv_fallbacks;
   int64_t train_samples;
   double est_var;              /* mean over timed blocks of sum w^2 (E_L - E)^2 / (sum w)^2 (Ha^2) */
+  int64_t shadow_n;            /* blocks with a shadow exact measurement                 */
+  double shadow_shift;         /* mean of (surrogate - exact FP64) block energy (Ha)     */
+  double shadow_var;           /* variance of that difference over the blocks (Ha^2)    */
+  double shadow_est_var;       /* est_var of the exact FP64 energies of the same blocks */
+  double shadow_wall_s;        /* wall seconds spent in shadow measurements (not timed) */
+  double live_frac;            /* mean fraction of live walkers (weight > 0) over the timed steps */
 } afq_run_result;
 
 /* ref_up/ref_dn: reference determinant (rhf.txt or uhf.txt by walker type,

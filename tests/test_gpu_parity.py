@@ -102,6 +102,7 @@ def build(name, W, ref_compat=False, dt=0.005):
     orc = O.Oracle(h, W, wrhf, ref_compat=ref_compat)
     ctx.prep_prop(ru, rd, dt)
     orc.prep_prop(ru, rd, dt)
+    ctx.ref_dets = (ru, rd)
     return h, wrhf, ctx, orc
 
 
@@ -150,7 +151,8 @@ def test_ham_and_overlap_dets(name, nchol_ene_frac):
 @pytest.mark.parametrize("name", list(SYSTEMS))
 def test_trajectory(name):
     """Replay the same draws through both sides: steps with QR every 3 steps,
-    energy measurement and comb every 4, one dead walker and one capped walker."""
+    energy measurement and comb every 4, one dead walker and one below the
+    liveness threshold; the energy shift is the trial's energy."""
     W = 6
     h, wrhf, ctx, orc = build(name, W)
     rng = np.random.default_rng(2)
@@ -161,7 +163,8 @@ def test_trajectory(name):
     ctx.set_walkers(dets, weights=weights)
     orc.set_walkers(dets, weights=weights)
     cap = 100.0
-    eshift = -1.0
+    o = orc.ham_and_overlap_dets(h.serial_det(*ctx.ref_dets)[None, :], h.nchol)[0]
+    eshift = (o[0] / o[1]).real
     nsteps = 8 if h.norb < 120 else 5
     for step in range(1, nsteps + 1):
         x = rng.standard_normal((W, h.nchol))
@@ -184,7 +187,7 @@ def test_trajectory(name):
             zeta = float(rng.uniform())
             tg = ctx.comb(zeta)
             to = orc.comb(zeta)
-            assert abs(tg - to) < TOL * abs(to)
+            assert to > 0 and abs(tg - to) < TOL * abs(to)
             check_state(h, ctx, orc, what="comb %d" % step)
         # force bias of the new state (used by the next step) matches too
         fbg = ctx.force_bias()
@@ -257,3 +260,41 @@ def test_device_rng_stream():
     ctx.propagate(None, -1.0, seed=7, step=4)
     d3, _, _ = ctx.get_walkers()
     assert not np.allclose(d1, d3)
+
+
+def test_largest_shape_parity():
+    """The benchmark's largest system (N=300, 60+60, G=1200, UHF trial) at full
+    shape: three steps with a QR, then force bias and local energy of each walker,
+    against the oracle on the same draws."""
+    from vmc_amd.synthetic import workload
+    h, spec = workload("c4")
+    W = 3
+    ctx = Context(h, W, False)
+    orc = O.Oracle(h, W, False)
+    ctx.prep_prop(h.phi_up, h.phi_dn, spec["dt"])
+    orc.prep_prop(h.phi_up, h.phi_dn, spec["dt"])
+    rng = np.random.default_rng(8)
+    dets = perturbed_walkers(h, W, False, rng, 0.02)
+    ctx.set_walkers(dets)
+    orc.set_walkers(dets)
+    o = orc.ham_and_overlap_dets(h.serial_det(h.phi_up, h.phi_dn)[None, :], h.nchol)[0]
+    eshift = (o[0] / o[1]).real
+    for step in range(1, 4):
+        x = rng.standard_normal((W, h.nchol))
+        fg, ncg = ctx.propagate(x, eshift)
+        fo, nco = orc.propagate(x, eshift, 100.0)
+        assert rel(fg, fo) < TOL, (step, fg, fo)
+        check_state(h, ctx, orc, what="c4 step %d" % step)
+    ctx.orthogonalize()
+    orc.orthogonalize()
+    check_state(h, ctx, orc, what="c4 qr")
+    fbg = ctx.force_bias()
+    e, ho = ctx.measure()
+    _, _, wo = orc.get_walkers()
+    for w in range(W):
+        assert rel(fbg[w], orc.force_bias(w)) < TOL, ("fb", w)
+        oo = orc.ham_and_overlap(w)
+        if wo[w] != 0:
+            assert rel_each(ho[w], oo) < TOL, ("hamAndOverlap", w)
+        else:  # no local energy for a dead walker; its overlap is still reported
+            assert ho[w][0] == 0 and e[w] == 0 and rel_each(ho[w][1:], oo[1:]) < TOL

@@ -9,7 +9,9 @@
 * the control-variate energy is unbiased: averaged over subset draws it equals
   the exact population energy, also with an untrained (bad) surrogate; with
   every walker in the exact set (risk fraction 1, or blocks of one) it equals
-  the exact energies walker by walker, and the fallback returns them too.
+  the exact energies walker by walker, and the fallback returns them too;
+* the reduced-precision exchange surrogate (one BF16 pass on every walker, FP64
+  on the exact set) is unbiased with a residual far below the spread of E_L.
 """
 import os
 import sys
@@ -18,7 +20,7 @@ import numpy as np
 import pytest
 
 from test_gpu_parity import build, perturbed_walkers
-from vmc_amd.abi import AFQ_ENERGY_SURROGATE
+from vmc_amd.abi import AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16, AFQ_SURR_EXCHANGE
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import mlp_oracle  # noqa: E402
@@ -81,7 +83,7 @@ def test_training_fits_teacher():
     h, wrhf, ctx, orc = build("c1_small", 64)
     F = h.nchol + 2 * h.norb
     rng = np.random.default_rng(11)
-    n = 8192
+    n = 32768
     X = rng.standard_normal((n, F)).astype(np.float32) * 3.0 + 1.0
     v1, v2 = rng.standard_normal(F), rng.standard_normal(F)
     v1 /= np.linalg.norm(v1)
@@ -92,7 +94,7 @@ def test_training_fits_teacher():
     assert ctx.surrogate_load(X, Y) == n
     tr, va = ctx.surrogate_train(epochs=30, batch=256, lr=1e-3, seed=13)
     print("teacher: sd %.3f  train rmse %.3f  valid rmse %.3f" % (Y.std(), tr, va))
-    assert va < 0.3 * Y.std() and tr < 0.3 * Y.std()
+    assert va < 0.25 * Y.std() and tr < 0.25 * Y.std()
 
 
 @pytest.fixture(scope="module")
@@ -124,12 +126,14 @@ def test_surrogate_on_walkers(trained):
     ej = 0.5 * np.sum((fb[:, :h.nchol] ** 2).real, axis=1)
     d = np.abs(ej - np.median(ej[live]))
     tame = live & (d <= np.quantile(d[live], 0.9))
-    rms = np.sqrt(np.mean((S - y)[tame] ** 2))
+    # the population drifts between training and use; the control variate tracks
+    # the mean residual separately, so the spread of the residual is what counts
+    rsd = (S - y)[tame].std()
     sd = y[tame].std()
-    print("walkers: train rmse %.3f valid rmse %.3f | fresh tame rms %.3f  sd(y) %.3f  sd(E_K) n/a"
-          % (tr, va, rms, sd))
+    print("walkers: train rmse %.3f valid rmse %.3f | fresh tame residual sd %.3f  sd(y) %.3f  "
+          "variance kept %.3f" % (tr, va, rsd, sd, (rsd / sd) ** 2))
     assert np.isfinite(S).all()
-    assert rms < 1.25 * sd
+    assert rsd < 1.1 * sd
 
 
 def test_control_variate_exact_limits(trained):
@@ -167,6 +171,62 @@ def test_control_variate_unbiased(trained, risk):
     assert abs(ests.mean() - E0) < 4 * sem + 1e-9
 
 
+def test_control_variate_clip(trained):
+    """with the population-control clip inside the estimate, the control variate
+    is unbiased for the clipped exact energies (the estimator the driver uses)"""
+    h, ctx, rng, tr, va = trained
+    grow_population(ctx, h, 2, rng)
+    e_exact, _ = ctx.measure()
+    w = ctx.weights()
+    live = w != 0
+    center = np.full(ctx.nwalk, np.median(e_exact[live]))
+    half = 0.5 * np.std(e_exact[live])  # clips a large share of the walkers
+    e_clip = np.where(live, np.clip(e_exact, center - half, center + half), 0.0)
+    E0 = np.sum(w * e_clip) / np.sum(w)
+    ctx.set_cv(subset_inv=1, risk_frac=0.0, seed=1)
+    ctx.set_energy_clip(center, half)
+    assert np.allclose(ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False), e_clip, rtol=0, atol=1e-9 * half)
+    ests = []
+    for seed in range(40):
+        ctx.set_cv(subset_inv=16, risk_frac=0.05, seed=700 + seed)
+        e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+        ests.append(np.sum(w * e) / np.sum(w))
+    ctx.set_energy_clip(None, 0.0)
+    ests = np.array(ests)
+    sem = ests.std(ddof=1) / np.sqrt(len(ests))
+    print("clipped: exact %.6f cv %.6f sem %.2e (%.0f%% of walkers clipped)"
+          % (E0, ests.mean(), sem, 100 * np.mean(np.abs(e_exact[live] - center[live]) > half)))
+    assert abs(ests.mean() - E0) < 4 * sem + 1e-9
+
+
+def test_trained_network_gain(trained):
+    """variance of the control-variate energy over exact-set draws with the trained
+    network against the same estimator with a constant surrogate (the method
+    without the network: plain subsampling around the mean of E_K - E_J)"""
+    h, ctx, rng, tr, va = trained
+    grow_population(ctx, h, 2, rng)
+    ctx.measure()
+    w = ctx.weights()
+    p_net = ctx.surrogate_params()
+    y = exact_y(ctx, h)
+    p_const = p_net.copy()
+    p_const[-2], p_const[-1] = np.sum(w * y) / np.sum(w), 0.0
+    out = {}
+    for name, p in (("network", p_net), ("constant", p_const)):
+        ctx.surrogate_params(p)
+        ests = []
+        for seed in range(40):
+            ctx.set_cv(subset_inv=16, risk_frac=0.05, seed=500 + seed)
+            e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+            ests.append(np.sum(w * e) / np.sum(w))
+        out[name] = np.var(ests, ddof=1)
+    ctx.surrogate_params(p_net)
+    gain = out["constant"] / out["network"]
+    print("subset-sampling variance: network %.4e  constant %.4e  gain %.3f"
+          % (out["network"], out["constant"], gain))
+    assert gain > 0.6
+
+
 def test_control_variate_unbiased_with_untrained_surrogate():
     W = 256
     h, wrhf, ctx, orc = build("c1_small", W)
@@ -187,3 +247,31 @@ def test_control_variate_unbiased_with_untrained_surrogate():
     ests = np.array(ests)
     sem = ests.std(ddof=1) / np.sqrt(len(ests))
     assert abs(ests.mean() - np.sum(w * e_exact) / np.sum(w)) < 4 * sem
+
+
+def test_exchange_surrogate():
+    W = 256
+    h, wrhf, ctx, orc = build("c1_small", W)
+    rng = np.random.default_rng(21)
+    ctx.set_walkers(perturbed_walkers(h, W, wrhf, rng, 0.02))
+    grow_population(ctx, h, 3, rng)
+    e_exact, _ = ctx.measure()
+    w = ctx.weights()
+    live = w != 0
+    E0 = np.sum(w * e_exact) / np.sum(w)
+    ctx.set_surrogate_kind(AFQ_SURR_EXCHANGE, AFQ_PREC_BF16)
+    ctx.set_cv(subset_inv=1, risk_frac=0.0, seed=1)
+    e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+    assert np.allclose(e, e_exact, rtol=0, atol=1e-9 * np.abs(e_exact).max())
+    rms = ctx.cv_stats()["rms"]
+    ests = []
+    for seed in range(40):
+        ctx.set_cv(subset_inv=16, risk_frac=0.0, seed=900 + seed)
+        e = ctx.measure(mode=AFQ_ENERGY_SURROGATE, want_ho=False)
+        ests.append(np.sum(w * e) / np.sum(w))
+    ests = np.array(ests)
+    sem = ests.std(ddof=1) / np.sqrt(len(ests))
+    print("bf16 exchange surrogate: residual rms %.3e Ha (sd E_L %.3e); exact %.8f cv %.8f sem %.2e"
+          % (rms, e_exact[live].std(), E0, ests.mean(), sem))
+    assert rms < 0.02 * e_exact[live].std()
+    assert abs(ests.mean() - E0) < 4 * sem + 1e-9

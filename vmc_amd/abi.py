@@ -15,6 +15,8 @@ AFQ_PREC_BF16 = 3
 AFQ_PREC_FP8 = 4
 AFQ_ENERGY_EXACT = 0
 AFQ_ENERGY_SURROGATE = 1
+AFQ_SURR_MLP = 0
+AFQ_SURR_EXCHANGE = 1
 
 DP = C.POINTER(C.c_double)
 

@@ -9,7 +9,8 @@ import os
 
 import numpy as np
 
-from .abi import (AFQ_ENERGY_EXACT, AFQ_PREC_FP64, AfqHam, AfqParams, AfqTrial, DP, dptr)
+from .abi import (AFQ_ENERGY_EXACT, AFQ_PREC_BF16, AFQ_PREC_FP64, AFQ_SURR_MLP, AfqHam, AfqParams, AfqTrial, DP,
+                  dptr)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvmc_afqmc.so")
@@ -61,6 +62,8 @@ def lib():
         L.afq_surrogate_params.argtypes = [vp, FP, i64, i32]
         L.afq_surrogate_grad.argtypes = [vp, FP, FP, i32, FP, DP]
         L.afq_set_cv.argtypes = [vp, i32, d, d, u64]
+        L.afq_set_energy_clip.argtypes = [vp, DP, d]
+        L.afq_set_surrogate_kind.argtypes = [vp, i32, i32]
         L.afq_cv_stats.argtypes = [vp, DP]
         L.afq_energy_parts.argtypes = [vp, DP, DP]
         L.afq_surrogate_features.argtypes = [vp, FP]
@@ -69,7 +72,8 @@ def lib():
     return _lib
 
 
-STAGES = ["force_bias", "fields", "vhs", "onebody", "green", "weight", "ortho", "exchange", "comb", "taylor"]
+STAGES = ["force_bias", "fields", "vhs", "onebody", "green", "weight", "ortho", "exchange", "comb", "taylor",
+          "exchange_lp"]
 
 
 def c2f(a):
@@ -217,7 +221,7 @@ class Context:
         return t[0]
 
     def stage_times(self, reset=True):
-        ms = np.zeros(10)
+        ms = np.zeros(len(STAGES))
         self._chk(lib().afq_stage_times(self.ptr, dptr(ms), int(reset)), "afq_stage_times")
         return dict(zip(STAGES, ms))
 
@@ -299,6 +303,13 @@ class Context:
     def set_cv(self, subset_inv=16, risk_frac=0.05, fallback_rms=0.0, seed=5):
         self._chk(lib().afq_set_cv(self.ptr, subset_inv, risk_frac, fallback_rms, seed), "afq_set_cv")
 
+    def set_surrogate_kind(self, kind, prec=AFQ_PREC_BF16):
+        self._chk(lib().afq_set_surrogate_kind(self.ptr, kind, prec), "afq_set_surrogate_kind")
+
+    def set_energy_clip(self, center=None, halfwidth=0.0):
+        c = None if center is None else np.ascontiguousarray(center, dtype=np.float64)
+        self._chk(lib().afq_set_energy_clip(self.ptr, dptr(c), halfwidth), "afq_set_energy_clip")
+
     def cv_stats(self):
         o = np.zeros(6)
         self._chk(lib().afq_cv_stats(self.ptr, dptr(o)), "afq_cv_stats")
@@ -320,7 +331,8 @@ class RunParams(C.Structure):
         ("energy_mode", C.c_int32), ("device", C.c_int32),
         ("surrogate_train_steps", C.c_int32), ("surrogate_hidden", C.c_int32),
         ("surrogate_epochs", C.c_int32), ("subset_inv", C.c_int32), ("fallback_rms", C.c_double),
-        ("risk_frac", C.c_double),
+        ("risk_frac", C.c_double), ("shadow_exact", C.c_int32), ("surrogate_kind", C.c_int32),
+        ("surrogate_prec", C.c_int32),
     ]
 
 
@@ -328,9 +340,11 @@ class RunResult(C.Structure):
     _fields_ = [("ref_energy", C.c_double), ("nchol_ene", C.c_int32), ("delta", C.c_double),
                 ("n_large_deviations", C.c_int64), ("prop_time", C.c_double),
                 ("ene_time", C.c_double), ("timed_steps", C.c_int64), ("timed_wall_s", C.c_double),
-                ("stage_ms", C.c_double * 10), ("train_rmse", C.c_double), ("valid_rmse", C.c_double),
+                ("stage_ms", C.c_double * len(STAGES)), ("train_rmse", C.c_double), ("valid_rmse", C.c_double),
                 ("cv_var_sub", C.c_double), ("cv_rms", C.c_double), ("cv_fallbacks", C.c_int64),
-                ("train_samples", C.c_int64), ("est_var", C.c_double)]
+                ("train_samples", C.c_int64), ("est_var", C.c_double), ("shadow_n", C.c_int64),
+                ("shadow_shift", C.c_double), ("shadow_var", C.c_double), ("shadow_est_var", C.c_double),
+                ("shadow_wall_s", C.c_double), ("live_frac", C.c_double)]
 
 
 def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, nsweeps, ortho_steps,
@@ -339,7 +353,8 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   rng_mode=AFQ_RNG_REPLAY, warmup_steps=0, prec_fb=AFQ_PREC_FP64,
                   prec_vhs=AFQ_PREC_FP64, prec_exchange=AFQ_PREC_FP64,
                   energy_mode=AFQ_ENERGY_EXACT, device=0, surrogate_train_steps=0, surrogate_hidden=256,
-                  surrogate_epochs=20, subset_inv=16, fallback_rms=0.0, risk_frac=0.05):
+                  surrogate_epochs=20, subset_inv=16, fallback_rms=0.0, risk_frac=0.05, shadow_exact=False,
+                  surrogate_kind=AFQ_SURR_MLP, surrogate_prec=AFQ_PREC_BF16):
     """calcMixedEstimatorLongProp on the GPU; returns (weights, energies, RunResult)."""
     L = lib()
     if not hasattr(L, "_run_bound"):
@@ -351,7 +366,7 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   int(ref_compat), cholesky_threshold, ene0_guess, weight_cap, max_steps, rng_mode,
                   warmup_steps, prec_fb, prec_vhs, prec_exchange, energy_mode, device,
                   surrogate_train_steps, surrogate_hidden, surrogate_epochs, subset_inv, fallback_rms,
-                  risk_frac)
+                  risk_frac, int(shadow_exact), surrogate_kind, surrogate_prec)
     w = np.zeros(nsweeps)
     e = np.zeros(nsweeps)
     res = RunResult()

@@ -101,12 +101,20 @@ struct afq_ctx {
   afq::Mlp mlp;
   int cv_minv = 16;
   double cv_risk = 0.05;
+  int cv_kind = AFQ_SURR_MLP;
+  int cv_prec = AFQ_PREC_BF16;   // precision of the AFQ_SURR_EXCHANGE level
+  double cv_offset = 0.0;   // running mean residual of earlier measurements, subtracted from S
+  double cv_offset_rate = 0.1;
   double cv_fallback = 0.0;
   std::mt19937_64 cv_rng{1};
   double* cvS = nullptr;
   double* cvEK = nullptr;
   double* cvEJ = nullptr;
   float* cvPinv = nullptr;
+  int* liveSel = nullptr;       // live walkers of the last measurement
+  double* cvD = nullptr;        // clipped exact - surrogate energy of the exact set
+  double* cvCenter = nullptr;   // per-walker clip centre (null: no clipping)
+  double cv_halfw = 0.0;
   int* cvSel = nullptr;
   float* cvX = nullptr;
   double cv_stats[6] = {0, 0, 0, 0, 0, 0};
@@ -335,9 +343,49 @@ void refresh_walkers(afq_ctx* c, bool overlaps_from_dets) {
   launch_check("force_bias");
 }
 
+// Indices of the live walkers (weight != 0), also uploaded to c->liveSel.  Dead
+// walkers get no local energy (DQMC/MixedEstimator.cpp:531), so the measurement
+// skips their exchange contraction.
+int live_list(afq_ctx* c, std::vector<int>& live) {
+  DevState& s = c->s;
+  std::vector<double> w(s.W);
+  download(w.data(), s.weights, s.W, c->stream);
+  live.clear();
+  for (int i = 0; i < s.W; ++i)
+    if (w[i] != 0.0) live.push_back(i);
+  if (!c->liveSel) c->liveSel = dalloc<int>(s.W);
+  if (!live.empty())
+    hip_check(hipMemcpyAsync(c->liveSel, live.data(), live.size() * sizeof(int), hipMemcpyHostToDevice, c->stream),
+              "H2D");
+  return (int)live.size();
+}
+
+void free_cv(afq_ctx* c) {
+  for (void* p : {(void*)c->cvS, (void*)c->cvEK, (void*)c->cvEJ, (void*)c->cvPinv, (void*)c->cvSel, (void*)c->cvX,
+                  (void*)c->cvD, (void*)c->cvCenter, (void*)c->liveSel})
+    if (p) (void)hipFree(p);
+  c->liveSel = nullptr;
+  c->cvS = c->cvEK = c->cvEJ = c->cvD = c->cvCenter = nullptr;
+  c->cvPinv = nullptr;
+  c->cvSel = nullptr;
+  c->cvX = nullptr;
+  c->cv_halfw = 0.0;
+}
+
+// buffers of the control variate (both surrogate kinds)
+void ensure_cv(afq_ctx* c) {
+  DevState& s = c->s;
+  if (!c->cvS) c->cvS = dalloc<double>(s.W);
+  if (!c->cvEK) c->cvEK = dalloc<double>(s.W);
+  if (!c->cvEJ) c->cvEJ = dalloc<double>(s.W);
+  if (!c->cvPinv) c->cvPinv = dalloc<float>(s.W);
+  if (!c->cvD) c->cvD = dalloc<double>(s.W);
+  if (!c->cvSel) c->cvSel = dalloc<int>(s.W);
+}
+
 void free_mlp(afq_ctx* c) {
   afq::Mlp& m = c->mlp;
-  std::vector<void*> ptrs = {m.mu, m.sd, m.lossbuf, m.X, m.Y, c->cvS, c->cvEK, c->cvEJ, c->cvPinv, c->cvSel, c->cvX};
+  std::vector<void*> ptrs = {m.mu, m.sd, m.lossbuf, m.X, m.Y, c->cvX};
   for (int l = 0; l < 4; ++l)
     for (void* p : {(void*)m.w[l], (void*)m.b[l], (void*)m.mw[l], (void*)m.vw[l], (void*)m.mb[l], (void*)m.vb[l],
                     (void*)m.gw[l], (void*)m.gb[l], (void*)m.z[l], (void*)m.h[l], (void*)m.dh[l]})
@@ -345,9 +393,6 @@ void free_mlp(afq_ctx* c) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   m = afq::Mlp();
-  c->cvS = c->cvEK = c->cvEJ = nullptr;
-  c->cvPinv = nullptr;
-  c->cvSel = nullptr;
   c->cvX = nullptr;
 }
 
@@ -493,6 +538,7 @@ int afq_destroy(afq_ctx* c) {
   for (auto& p : c->pool) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   free_all(c->s);
   free_mlp(c);
+  free_cv(c);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return AFQ_OK;
@@ -715,10 +761,26 @@ static void risk_split(afq_ctx* c, std::vector<char>& live, std::vector<char>& r
 static int measure_cv(afq_ctx* c, double delta, double* eloc) {
   DevState& s = c->s;
   afq::Mlp& m = c->mlp;
-  if (!m.ready) return fail(c, AFQ_ERR_STATE, "surrogate not trained");
   const int mm = c->cv_minv;
-  afq::launch_features(s, nullptr, s.W, c->cvX, 0, c->stream);
-  afq::launch_predict(s, m, c->cvX, s.W, c->cvS, c->stream);
+  if (c->cv_kind == AFQ_SURR_MLP) {
+    if (!m.ready) return fail(c, AFQ_ERR_STATE, "surrogate not trained");
+    afq::launch_features(s, nullptr, s.W, c->cvX, 0, c->stream);
+    afq::launch_predict(s, m, c->cvX, s.W, c->cvS, c->stream);
+  } else {
+    // reduced-precision exchange of every live walker: S = E_K(reduced) - E_J
+    ensure_cv(c);
+    std::vector<int> lv;
+    const int nl = live_list(c, lv);
+    if (s.GE > 0 && nl > 0) {
+      Stage t(c, afq::ST_EXLP);
+      afq::launch_exchange_lp(s, c->liveSel, nl, c->cv_prec, c->stream);
+      launch_check("exchange (surrogate level)");
+    }
+    if (nl > 0) afq::launch_ek(s, c->liveSel, nl, c->cvS, nullptr, 0, c->stream);
+    afq::launch_ej(s, c->cvEJ, c->stream);
+    afq::launch_sub(c->cvS, c->cvEJ, s.W, c->stream);
+    launch_check("surrogate level");
+  }
   std::vector<char> live, risky;
   risk_split(c, live, risky);
   std::vector<int> sel, tame;
@@ -746,10 +808,13 @@ static int measure_cv(afq_ctx* c, double delta, double* eloc) {
     launch_check("exchange (subset)");
   }
   if (nsel > 0) afq::launch_ek(s, c->cvSel, nsel, c->cvEK, nullptr, 0, c->stream);
-  afq::launch_energy_cv(s, c->cvS, c->cvEK, c->cvPinv, delta, c->stream);
+  const double soff = c->cv_offset;
+  afq::launch_energy_cv(s, c->cvS, c->cvEK, c->cvPinv, delta, soff, c->cv_halfw > 0.0 ? c->cvCenter : nullptr,
+                        c->cv_halfw, c->cvD, c->stream);
   launch_check("energy_cv");
   // residual statistics of the random part of the sample
-  std::vector<double> S(s.W), EK(s.W), EJ(s.W), w(s.W);
+  std::vector<double> S(s.W), EK(s.W), EJ(s.W), w(s.W), D(s.W);
+  download(D.data(), c->cvD, s.W, c->stream);
   download(S.data(), c->cvS, s.W, c->stream);
   download(EK.data(), c->cvEK, s.W, c->stream);
   download(EJ.data(), c->cvEJ, s.W, c->stream);
@@ -758,20 +823,22 @@ static int measure_cv(afq_ctx* c, double delta, double* eloc) {
   for (int i = 0; i < s.W; ++i) wsum += w[i];
   for (int q = nrisk; q < nsel; ++q) {
     const int k = sel[q];
-    const double r = S[k] - (EK[k] - EJ[k]);
+    const double r = S[k] - soff - (EK[k] - EJ[k]);
     r2 += r * r;
     rm += r;
-    wr2 += (double)pinv[k] * (pinv[k] - 1.0) * (w[k] * r) * (w[k] * r);
+    wr2 += (double)pinv[k] * (pinv[k] - 1.0) * (w[k] * D[k]) * (w[k] * D[k]);
   }
   const int nrand = nsel - nrisk;
   const double rms = nrand ? std::sqrt(r2 / nrand) : 0.0;
   bool fallback = false;
   if (c->cv_fallback > 0.0 && rms > c->cv_fallback) {
     fallback = true;
-    if (s.GE > 0) {
+    std::vector<int> lv;
+    const int nl = live_list(c, lv);
+    if (s.GE > 0 && nl > 0) {
       Stage t(c, afq::ST_ENERGY);
-      if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, nullptr, s.W, c->stream);
-      else afq::launch_exchange_lp(s, nullptr, s.W, s.prec_exchange, c->stream);
+      if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, c->liveSel, nl, c->stream);
+      else afq::launch_exchange_lp(s, c->liveSel, nl, s.prec_exchange, c->stream);
     }
     afq::launch_energy(s, nullptr, s.W, delta, c->stream);
     launch_check("fallback exact");
@@ -781,6 +848,8 @@ static int measure_cv(afq_ctx* c, double delta, double* eloc) {
   c->cv_stats[2] = fallback ? 1.0 : 0.0;
   c->cv_stats[3] = fallback || wsum <= 0 ? 0.0 : wr2 / (wsum * wsum);
   c->cv_stats[4] = nrand ? rm / nrand : 0.0;
+  // the offset only ever uses residuals of earlier draws, so the next estimate stays unbiased
+  if (nrand && !fallback) c->cv_offset += c->cv_offset_rate * (rm / nrand);
   c->cv_stats[5] += fallback ? 1.0 : 0.0;
   c->last_exact_all = fallback;
   if (eloc) download(eloc, s.eloc, s.W, c->stream);
@@ -797,10 +866,13 @@ int afq_measure(afq_ctx* c, int32_t mode, double delta, double* eloc, double* ha
     if (ham_ovlp) return fail(c, AFQ_ERR_ARG, "ham_ovlp is not produced by the surrogate mode");
     return measure_cv(c, delta, eloc);
   }
-  if (s.GE > 0) {
+  std::vector<int> lv;
+  const int nl = live_list(c, lv);
+  const int* sel = nl < s.W ? c->liveSel : nullptr;
+  if (s.GE > 0 && nl > 0) {
     Stage t(c, afq::ST_ENERGY);  // the exchange contraction alone
-    if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, nullptr, s.W, c->stream);
-    else afq::launch_exchange_lp(s, nullptr, s.W, s.prec_exchange, c->stream);
+    if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, sel, nl, c->stream);
+    else afq::launch_exchange_lp(s, sel, nl, s.prec_exchange, c->stream);
     launch_check("exchange");
   }
   afq::launch_energy(s, nullptr, s.W, delta, c->stream);
@@ -864,6 +936,7 @@ int afq_comb(afq_ctx* c, double zeta, double* total_weight) {
 }
 
 int afq_stage_times(afq_ctx* c, double* ms10, int32_t reset) {
+  static_assert(afq::ST_COUNT == AFQ_NSTAGES, "stage count");
   if (!c) return AFQ_ERR_ARG;
   AFQ_GUARD_BEGIN
   c->resolve();
@@ -976,11 +1049,7 @@ int afq_surrogate_init(afq_ctx* c, int32_t hidden, uint64_t seed, int64_t capaci
     m.X = dalloc<float>(capacity * F);
     m.Y = dalloc<float>(capacity);
   }
-  c->cvS = dalloc<double>(s.W);
-  c->cvEK = dalloc<double>(s.W);
-  c->cvEJ = dalloc<double>(s.W);
-  c->cvPinv = dalloc<float>(s.W);
-  c->cvSel = dalloc<int>(s.W);
+  ensure_cv(c);
   c->cvX = dalloc<float>((int64_t)s.W * F);
   return AFQ_OK;
   AFQ_GUARD_END(c)
@@ -1041,11 +1110,14 @@ int afq_surrogate_train(afq_ctx* c, int32_t epochs, int32_t batch, double lr, ui
   std::vector<float> X((size_t)n * F), Y(n);
   hip_check(scopy(c, X.data(), m.X, X.size() * 4, hipMemcpyDeviceToHost), "D2H");
   hip_check(scopy(c, Y.data(), m.Y, Y.size() * 4, hipMemcpyDeviceToHost), "D2H");
+  // validation = the most recent tenth of the samples (collection order): the
+  // same walker appears in consecutive steps with nearly the same features and
+  // target, so a random split would hold out near-copies of training samples
   std::vector<int> perm(n);
   std::iota(perm.begin(), perm.end(), 0);
   std::mt19937_64 rng(seed);
-  std::shuffle(perm.begin(), perm.end(), rng);
   const int64_t nval = std::max<int64_t>(1, n / 10), ntr = n - nval;
+  std::shuffle(perm.begin(), perm.begin() + ntr, rng);
   std::vector<double> mu(F, 0.0), var(F, 0.0);
   double ym = 0.0, yv = 0.0;
   for (int64_t q = 0; q < ntr; ++q) {
@@ -1079,11 +1151,12 @@ int afq_surrogate_train(afq_ctx* c, int32_t epochs, int32_t batch, double lr, ui
   };
   try {
     hip_check(scopy(c, didx, perm.data(), n * sizeof(int), hipMemcpyHostToDevice), "H2D");
-    // RMSE (Ha) of the prediction over cnt samples listed at idx (device)
-    auto rmse = [&](const int* idx, int64_t cnt) {
-      double acc = 0.0;
+    // predictions (Ha) and targets of cnt samples listed at idx (device)
+    auto predict = [&](const int* idx, int64_t cnt, std::vector<double>& pred, std::vector<double>& targ) {
       std::vector<float> y(std::min<int64_t>(cnt, m.Bcap));
       std::vector<int> ih(cnt);
+      pred.resize(cnt);
+      targ.resize(cnt);
       hip_check(scopy(c, ih.data(), idx, cnt * sizeof(int), hipMemcpyDeviceToHost), "D2H");
       for (int64_t b0 = 0; b0 < cnt; b0 += m.Bcap) {
         const int B = (int)std::min<int64_t>(m.Bcap, cnt - b0);
@@ -1091,11 +1164,16 @@ int afq_surrogate_train(afq_ctx* c, int32_t epochs, int32_t batch, double lr, ui
         hip_check(hipMemcpyAsync(y.data(), m.h[3], B * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "sync");
         for (int q = 0; q < B; ++q) {
-          const double pred = (double)y[q] * m.t_sd + m.t_mu;
-          const double d = pred - Y[ih[b0 + q]];
-          acc += d * d;
+          pred[b0 + q] = (double)y[q] * m.t_sd + m.t_mu;
+          targ[b0 + q] = Y[ih[b0 + q]];
         }
       }
+    };
+    auto rmse = [&](const int* idx, int64_t cnt) {
+      std::vector<double> p, t;
+      predict(idx, cnt, p, t);
+      double acc = 0.0;
+      for (int64_t q = 0; q < cnt; ++q) acc += (p[q] - t[q]) * (p[q] - t[q]);
       return std::sqrt(acc / cnt);
     };
     // the weights of the epoch with the lowest validation error are kept
@@ -1129,8 +1207,28 @@ int afq_surrogate_train(afq_ctx* c, int32_t epochs, int32_t batch, double lr, ui
       }
     }
     snapshot(true);
+    // Calibration on the validation set: the output becomes a + b * (network),
+    // b = max(0, cov(y, p) / var(p)), the least-squares control-variate
+    // coefficient; a network that predicts nothing falls back to the constant a.
+    {
+      std::vector<double> p, t;
+      predict(didx + ntr, nval, p, t);
+      double pm = 0.0, tm = 0.0, cpt = 0.0, vp = 0.0;
+      for (int64_t q = 0; q < nval; ++q) { pm += p[q]; tm += t[q]; }
+      pm /= nval;
+      tm /= nval;
+      for (int64_t q = 0; q < nval; ++q) {
+        cpt += (p[q] - pm) * (t[q] - tm);
+        vp += (p[q] - pm) * (p[q] - pm);
+      }
+      const double bcal = vp > 0.0 ? std::max(0.0, cpt / vp) : 0.0;
+      const double acal = tm - bcal * pm;
+      m.t_mu = (float)(acal + bcal * m.t_mu);
+      m.t_sd = (float)(bcal * m.t_sd);
+      m.calib_b = bcal;
+    }
     if (train_rmse) *train_rmse = rmse(didx, ntr);
-    if (valid_rmse) *valid_rmse = best_va;
+    if (valid_rmse) *valid_rmse = rmse(didx + ntr, nval);
     hip_check(hipStreamSynchronize(c->stream), "sync");
   } catch (...) {
     release();
@@ -1252,9 +1350,39 @@ int afq_set_cv(afq_ctx* c, int32_t subset_inv, double risk_frac, double fallback
   if (!(risk_frac >= 0.0 && risk_frac <= 1.0)) return fail(c, AFQ_ERR_ARG, "risk_frac must lie in [0, 1]");
   c->cv_minv = subset_inv;
   c->cv_risk = risk_frac;
+  c->cv_offset = 0.0;
   c->cv_fallback = fallback_rms;
   c->cv_rng.seed(seed);
   return AFQ_OK;
+}
+
+int afq_set_surrogate_kind(afq_ctx* c, int32_t kind, int32_t prec) {
+  if (!c) return AFQ_ERR_ARG;
+  if (kind != AFQ_SURR_MLP && kind != AFQ_SURR_EXCHANGE) return fail(c, AFQ_ERR_ARG, "unknown surrogate kind");
+  if (kind == AFQ_SURR_EXCHANGE && prec != AFQ_PREC_BF16 && prec != AFQ_PREC_BF16X3)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "surrogate exchange precision must be BF16 or BF16X3");
+  AFQ_GUARD_BEGIN
+  if (kind == AFQ_SURR_EXCHANGE) ensure_lowprec(c->s, c->stream);
+  ensure_cv(c);
+  c->cv_kind = kind;
+  c->cv_prec = prec;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_set_energy_clip(afq_ctx* c, const double* center, double halfwidth) {
+  if (!c) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  ensure_cv(c);
+  if (!center || !(halfwidth > 0.0)) {
+    c->cv_halfw = 0.0;
+    return AFQ_OK;
+  }
+  if (!c->cvCenter) c->cvCenter = dalloc<double>(c->s.W);
+  hip_check(scopy(c, c->cvCenter, center, c->s.W * sizeof(double), hipMemcpyHostToDevice), "H2D");
+  c->cv_halfw = halfwidth;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
 }
 
 int afq_cv_stats(afq_ctx* c, double* out6) {

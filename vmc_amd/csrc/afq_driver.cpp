@@ -125,14 +125,18 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   int64_t timed = 0;
   afq_stage_times(c, nullptr, 1);
   const bool cv = p->energy_mode == AFQ_ENERGY_SURROGATE;
-  const int ntrain = cv ? std::max(1, p->surrogate_train_steps) : 0;
+  const bool mlp = cv && p->surrogate_kind == AFQ_SURR_MLP;
+  const int ntrain = mlp ? std::max(1, p->surrogate_train_steps) : 0;
   double cv_var = 0.0, cv_rms = 0.0;
   int64_t cv_n = 0, nsamples = 0;
   double tr_rmse = 0.0, va_rmse = 0.0;
   if (cv) {
-    const int hidden = p->surrogate_hidden > 0 ? p->surrogate_hidden : 256;
-    const int64_t cap = (int64_t)ntrain * W;
-    if ((rc = afq_surrogate_init(c, hidden, (uint64_t)p->seed + 17, cap))) return rc;
+    if (mlp) {
+      const int hidden = p->surrogate_hidden > 0 ? p->surrogate_hidden : 256;
+      const int64_t cap = (int64_t)ntrain * W;
+      if ((rc = afq_surrogate_init(c, hidden, (uint64_t)p->seed + 17, cap))) return rc;
+    }
+    if ((rc = afq_set_surrogate_kind(c, p->surrogate_kind, p->surrogate_prec))) return rc;
     if ((rc = afq_set_cv(c, p->subset_inv > 0 ? p->subset_inv : 16, p->risk_frac, p->fallback_rms,
                          (uint64_t)p->seed + 29)))
       return rc;
@@ -140,6 +144,36 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   int64_t nmeasured = 0;
   double est_var = 0.0;
   int64_t est_n = 0;
+  // shadow exact measurements of the surrogate blocks
+  std::vector<double> eloc_sh(W), centers(W);
+  double sh_sum = 0.0, sh_sum2 = 0.0, sh_est = 0.0, sh_wall = 0.0;
+  double sh_stage[AFQ_NSTAGES] = {0};
+  int64_t sh_n = 0;
+  double live_sum = 0.0;
+  const double clip_w = std::sqrt(2. / dt);
+  // DQMC/MixedEstimator.cpp:525-530: clip to the rank's running average -+ sqrt(2/dt)
+  auto clip_host = [&](std::vector<double>& e, bool count) {
+    for (int r = 0; r < R; ++r)
+      for (int wl = 0; wl < nw; ++wl) {
+        const int w = r * nw + wl;
+        if (weights[w] == 0.) { e[w] = 0.; continue; }
+        if (std::isnan(e[w]) || std::isinf(e[w])) return false;
+        if (std::fabs(e[w] - averageEnergy[r]) > clip_w) {
+          if (count) ++nLarge;
+          e[w] = e[w] > averageEnergy[r] ? averageEnergy[r] + clip_w : averageEnergy[r] - clip_w;
+        }
+      }
+    return true;
+  };
+  auto block_mean = [&](const std::vector<double>& e, double& var) {
+    double we = 0.0, tw = 0.0;
+    for (int w = 0; w < W; ++w) { we += e[w] * weights[w]; tw += weights[w]; }
+    const double m = we / tw;
+    double v = 0.0;
+    for (int w = 0; w < W; ++w) v += weights[w] * weights[w] * (e[w] - m) * (e[w] - m);
+    var = v / (tw * tw);
+    return m;
+  };
   const int64_t last = (int64_t)nsweeps * nsteps;
   for (int64_t step = 1; step < last; ++step) {
     if (p->max_steps > 0 && step > p->max_steps) break;
@@ -168,7 +202,12 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     if ((rc = afq_get_weights(c, weights.data()))) return rc;
     propTime += now_s() - t0;
     totalWeight = 0.0;
-    for (int w = 0; w < W; ++w) totalWeight += weights[w];
+    int nlive = 0;
+    for (int w = 0; w < W; ++w) {
+      totalWeight += weights[w];
+      nlive += weights[w] > 0.0;
+    }
+    if (step > p->warmup_steps) live_sum += (double)nlive / W;
     eshift = eEstimate - 0.1 * std::log(totalWeight / (double)W) / dt;
     if (step % p->ortho_steps == 0)
       if ((rc = afq_orthogonalize(c))) return rc;
@@ -178,11 +217,46 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
       const bool exact_now = !cv || nmeasured < ntrain;
       // training pairs come from the FP64 path whatever the production precision
       if (cv && exact_now && (rc = afq_set_precision(c, AFQ_PREC_FP64, AFQ_PREC_FP64, AFQ_PREC_FP64))) return rc;
+      if (!exact_now) {  // the surrogate estimate clips before its correction
+        for (int w = 0; w < W; ++w) centers[w] = averageEnergy[w / nw];
+        if ((rc = afq_set_energy_clip(c, centers.data(), clip_w))) return rc;
+      }
       if ((rc = afq_measure(c, exact_now ? AFQ_ENERGY_EXACT : AFQ_ENERGY_SURROGATE, delta, eloc.data(), nullptr)))
         return rc;
       if (cv && exact_now && (rc = afq_set_precision(c, p->prec_fb, p->prec_vhs, p->prec_exchange))) return rc;
+      bool clipped_on_device = false;
+      if (!exact_now) {
+        double st[6];
+        afq_cv_stats(c, st);
+        clipped_on_device = st[2] == 0.0;  // no fallback to the exact path
+      }
       eneTime += now_s() - t0;
       ++nmeasured;
+      if (!exact_now && p->shadow_exact) {
+        // exact FP64 energies of the same walkers, outside the timings
+        if ((rc = afq_sync(c))) return rc;
+        const double ts = now_s();
+        double before[AFQ_NSTAGES], after[AFQ_NSTAGES];
+        afq_stage_times(c, before, 0);
+        if ((rc = afq_set_precision(c, AFQ_PREC_FP64, AFQ_PREC_FP64, AFQ_PREC_FP64))) return rc;
+        if ((rc = afq_measure(c, AFQ_ENERGY_EXACT, delta, eloc_sh.data(), nullptr))) return rc;
+        if ((rc = afq_set_precision(c, p->prec_fb, p->prec_vhs, p->prec_exchange))) return rc;
+        afq_stage_times(c, after, 0);
+        if (step > p->warmup_steps)
+          for (int i = 0; i < AFQ_NSTAGES; ++i) sh_stage[i] += after[i] - before[i];
+        if (!clip_host(eloc_sh, false)) return rc = AFQ_ERR_NUMERIC;
+        std::vector<double> ecv = eloc;
+        if (!clipped_on_device && !clip_host(ecv, false)) return rc = AFQ_ERR_NUMERIC;
+        double v_sh, v_cv;
+        const double d = block_mean(ecv, v_cv) - block_mean(eloc_sh, v_sh);
+        if (step > p->warmup_steps) {
+          sh_sum += d;
+          sh_sum2 += d * d;
+          sh_est += v_sh;
+          ++sh_n;
+        }
+        if (step > p->warmup_steps) sh_wall += now_s() - ts;
+      }
       if (cv && exact_now) {
         if ((rc = afq_surrogate_collect(c, &nsamples))) return rc;
         if (nmeasured == ntrain) {
@@ -196,17 +270,14 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
         cv_rms += st[1];
         ++cv_n;
       }
-      for (int r = 0; r < R; ++r)
-        for (int wl = 0; wl < nw; ++wl) {
-          const int w = r * nw + wl;
-          if (weights[w] == 0.) { eloc[w] = 0.; continue; }
-          if (std::isnan(eloc[w]) || std::isinf(eloc[w])) return rc = AFQ_ERR_NUMERIC;
-          if (std::fabs(eloc[w] - averageEnergy[r]) > std::sqrt(2. / dt)) {
-            ++nLarge;
-            eloc[w] = eloc[w] > averageEnergy[r] ? averageEnergy[r] + std::sqrt(2. / dt)
-                                                 : averageEnergy[r] - std::sqrt(2. / dt);
-          }
+      if (clipped_on_device) {
+        for (int w = 0; w < W; ++w) {
+          if (weights[w] == 0.) eloc[w] = 0.;
+          else if (std::isnan(eloc[w]) || std::isinf(eloc[w])) return rc = AFQ_ERR_NUMERIC;
         }
+      } else if (!clip_host(eloc, true)) {
+        return rc = AFQ_ERR_NUMERIC;
+      }
       double we = 0.0;
       for (int w = 0; w < W; ++w) we += eloc[w] * weights[w];
       totalWeights[block] = totalWeight;
@@ -253,8 +324,9 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     res->prop_time = propTime;
     res->ene_time = eneTime;
     res->timed_steps = timed;
-    res->timed_wall_s = timed > 0 ? tEnd - tStart : 0.0;
+    res->timed_wall_s = timed > 0 ? tEnd - tStart - sh_wall : 0.0;
     afq_stage_times(c, res->stage_ms, 1);
+    for (int i = 0; i < AFQ_NSTAGES; ++i) res->stage_ms[i] -= sh_stage[i];
     res->train_rmse = tr_rmse;
     res->valid_rmse = va_rmse;
     res->cv_var_sub = cv_n ? cv_var / cv_n : 0.0;
@@ -264,6 +336,12 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     res->cv_fallbacks = (int64_t)st[5];
     res->train_samples = nsamples;
     res->est_var = est_n ? est_var / est_n : 0.0;
+    res->shadow_n = sh_n;
+    res->shadow_shift = sh_n ? sh_sum / sh_n : 0.0;
+    res->shadow_var = sh_n > 1 ? (sh_sum2 - sh_sum * sh_sum / sh_n) / (sh_n - 1) : 0.0;
+    res->shadow_est_var = sh_n ? sh_est / sh_n : 0.0;
+    res->shadow_wall_s = sh_wall;
+    res->live_frac = timed ? live_sum / timed : 0.0;
   }
   return AFQ_OK;
 }

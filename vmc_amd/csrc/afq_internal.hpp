@@ -24,7 +24,7 @@ struct Slot {
 
 // Per-kernel device time buckets (afq_stage_times order).
 enum Stage { ST_FB = 0, ST_FIELDS, ST_VHS, ST_PROP, ST_GREEN, ST_WEIGHT, ST_ORTHO, ST_ENERGY, ST_COMB, ST_TAYLOR,
-             ST_COUNT };
+             ST_EXLP, ST_COUNT };
 
 struct DevState {
   // sizes
@@ -97,6 +97,7 @@ struct Mlp {
   float* lossbuf = nullptr;
   int Bcap = 0;
   int t = 0;
+  double calib_b = 1.0;                     // calibration slope of the last training
   float* X = nullptr; float* Y = nullptr;   // features, target E_K - E_J
   int64_t n = 0, cap = 0;
   bool ready = false;
@@ -109,8 +110,9 @@ void launch_features(const DevState& s, const int* idx, int n, float* X, int64_t
 void launch_ek(const DevState& s, const int* idx, int n, double* ek, float* yout, int64_t dst0, hipStream_t st);
 void launch_predict(const DevState& s, const Mlp& m, const float* X, int n, double* S, hipStream_t st);
 void launch_ej(const DevState& s, double* ej, hipStream_t st);
+void launch_sub(double* a, const double* b, int n, hipStream_t st);  // a -= b
 void launch_energy_cv(const DevState& s, const double* S, const double* ek, const float* pinv, double delta,
-                      hipStream_t st);
+                      double soff, const double* center, double halfw, double* dres, hipStream_t st);
 
 // kernel launchers (afq_kernels.hip)
 void launch_force_bias(const DevState& s, hipStream_t st);

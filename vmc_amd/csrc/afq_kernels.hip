@@ -896,8 +896,6 @@ void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, h
   else {
     go(0, 1, s.Xp);
     // the second slot alone: present it as s0 with its own offsets
-    DevState t = s;
-    (void)t;
     const Slot& b = s.slot[1];
     const int gt = exchange_tile_g(b.n), wt = exchange_tile_w(b.n);
     const int ngt = (int)cdivu(s.GE, gt), gy = (int)cdivu(nsel, wt);
@@ -956,7 +954,7 @@ __global__ __launch_bounds__(64) void k_energy(const double* fb, const double* e
     eloc[w] = live ? er + delta : 0.0;
     if (ho) {
       const cplx O = cexp(cplx{logO[2 * w], logO[2 * w + 1]});
-      const cplx EO = cmul(cplx{er, ei}, O);
+      const cplx EO = live ? cmul(cplx{er, ei}, O) : cplx{0.0, 0.0};  // dead: no exchange was computed
       ho[4 * w] = EO.re; ho[4 * w + 1] = EO.im; ho[4 * w + 2] = O.re; ho[4 * w + 3] = O.im;
     }
   }

@@ -190,14 +190,18 @@ __global__ __launch_bounds__(64) void k_ej(const double* fb, double* ej, int G, 
   if (lane == 0) ej[w] = 0.5 * jr;
 }
 
-// control-variate local energy.  S_w predicts y = E_K - E_J, so the surrogate
-// estimate is E_w = ecore + e1 - S_w; a walker in the exact set (inclusion
-// probability 1/pinv[w]) adds pinv[w] (S_w - (E_K - E_J)), which makes the
-// estimate unbiased (Horvitz-Thompson).
+// control-variate local energy.  S_w - soff predicts y = E_K - E_J, so the
+// surrogate energy is e_s = ecore + e1 - (S_w - soff) + delta and the exact one
+// e_x = ecore + e1 - y + delta; both are clipped to center[w] -+ halfw (halfw > 0)
+// as the population-control estimator clips every local energy.  A walker in the
+// exact set (inclusion probability 1/pinv[w]) reports e_s + pinv[w] (e_x - e_s),
+// an unbiased (Horvitz-Thompson) estimate of its clipped exact energy for any soff
+// fixed before the exact set is drawn; dres[w] = e_x - e_s for the exact set.
 __global__ __launch_bounds__(64) void k_energy_cv(const double* fb, const double* e1, const double* S,
                                                   const double* ek, const float* pinv,
-                                                  const double* weights, double* eloc, int G, int GE, int nslots,
-                                                  int m0, int m1, double ecore, double delta) {
+                                                  const double* weights, double* eloc, double* dres, int G, int GE,
+                                                  int nslots, int m0, int m1, double ecore, double delta, double soff,
+                                                  const double* center, double halfw) {
   const int w = blockIdx.x, lane = threadIdx.x;
   double jr = 0.0;
   for (int g = lane; g < GE; g += 64) {
@@ -206,10 +210,20 @@ __global__ __launch_bounds__(64) void k_energy_cv(const double* fb, const double
   }
   jr = wave_sum(jr);
   if (lane == 0) {
-    double er = ecore - S[w];
-    for (int s = 0; s < nslots; ++s) er += (s == 0 ? m0 : m1) * e1[2 * ((int64_t)w * nslots + s)];
-    if (pinv[w] > 0.f) er += (double)pinv[w] * (S[w] - (ek[w] - 0.5 * jr));
-    eloc[w] = (weights == nullptr || weights[w] != 0.0) ? er + delta : 0.0;
+    const double lo = center && halfw > 0.0 ? center[w] - halfw : -INFINITY;
+    const double hi = center && halfw > 0.0 ? center[w] + halfw : INFINITY;
+    double base = ecore + delta;
+    for (int s = 0; s < nslots; ++s) base += (s == 0 ? m0 : m1) * e1[2 * ((int64_t)w * nslots + s)];
+    const double es = fmin(fmax(base - (S[w] - soff), lo), hi);
+    double er = es, d = 0.0;
+    if (pinv[w] > 0.f) {
+      const double ex = fmin(fmax(base - (ek[w] - 0.5 * jr), lo), hi);
+      d = ex - es;
+      er += (double)pinv[w] * d;
+    }
+    const bool live = weights == nullptr || weights[w] != 0.0;
+    eloc[w] = live ? er : 0.0;
+    dres[w] = live ? d : 0.0;
   }
 }
 
@@ -280,6 +294,15 @@ void launch_ek(const DevState& s, const int* idx, int n, double* ek, float* yout
                      s.slot[0].mult, s.nslots > 1 ? s.slot[1].mult : 0, ng0, ng1, s.ngt_cap, s.G, s.GE);
 }
 
+__global__ void k_sub(double* a, const double* b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] -= b[i];
+}
+
+void launch_sub(double* a, const double* b, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_sub, dim3(cdivu(n, 256)), dim3(256), 0, st, a, b, n);
+}
+
 void launch_ej(const DevState& s, double* ej, hipStream_t st) {
   hipLaunchKernelGGL(k_ej, dim3(s.W), dim3(64), 0, st, s.fb, ej, s.G, s.GE);
 }
@@ -291,9 +314,10 @@ void launch_predict(const DevState& s, const Mlp& m, const float* X, int n, doub
 }
 
 void launch_energy_cv(const DevState& s, const double* S, const double* ek, const float* pinv, double delta,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_energy_cv, dim3(s.W), dim3(64), 0, st, s.fb, s.e1, S, ek, pinv, s.weights, s.eloc, s.G, s.GE,
-                     s.nslots, s.slot[0].mult, s.nslots > 1 ? s.slot[1].mult : 0, s.ecore, delta);
+                      double soff, const double* center, double halfw, double* dres, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy_cv, dim3(s.W), dim3(64), 0, st, s.fb, s.e1, S, ek, pinv, s.weights, s.eloc, dres, s.G,
+                     s.GE, s.nslots, s.slot[0].mult, s.nslots > 1 ? s.slot[1].mult : 0, s.ecore, delta, soff, center,
+                     halfw);
 }
 
 }  // namespace afq
