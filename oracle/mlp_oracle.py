@@ -18,6 +18,9 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+_bf16 = bf16
+
+
 def gelu(z):
     return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
 
@@ -43,7 +46,10 @@ def unpack(params, dims):
     return Ws, bs, mu, sd, tmu, tsd
 
 
-def forward(params, dims, X):
+def forward(params, dims, X, rnd=None):
+    """rnd: operand rounding of the GEMMs (default bf16, as the GPU; identity for
+    the finite-difference check of the algebra)"""
+    bf16 = _bf16 if rnd is None else rnd
     Ws, bs, mu, sd, tmu, tsd = unpack(params, dims)
     xin = (np.asarray(X, np.float32) - mu.astype(np.float32)) / sd.astype(np.float32)
     ins, zs = [], []
@@ -56,9 +62,10 @@ def forward(params, dims, X):
     return h[:, 0], ins, zs
 
 
-def loss_and_grads(params, dims, X, t):
+def loss_and_grads(params, dims, X, t, rnd=None):
+    bf16 = _bf16 if rnd is None else rnd
     Ws, bs, mu, sd, tmu, tsd = unpack(params, dims)
-    y, ins, zs = forward(params, dims, X)
+    y, ins, zs = forward(params, dims, X, rnd)
     B = len(y)
     ts = (np.asarray(t, np.float64) - tmu) / tsd
     d = y - ts
