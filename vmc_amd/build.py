@@ -1,4 +1,5 @@
-"""Builds the HIP library vmc_amd/libvmc_afqmc.so for gfx950 (in-tree).
+"""Builds the HIP library vmc_amd/libvmc_afqmc.so for gfx950 (in-tree), and the
+host test programs that use it through include/vmc_afqmc.hpp.
 
     python -m vmc_amd.build            # compile changed sources, link
     python -m vmc_amd.build --force    # recompile everything
@@ -19,7 +20,8 @@ LIB = os.path.join(HERE, "libvmc_afqmc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 SOURCES = ["afq_kernels.hip", "afq_mlp.hip", "afq_ctx.cpp", "afq_driver.cpp"]
-HEADERS = ["afq_device.hpp", "afq_internal.hpp", "afq_driver.hpp"]
+HEADERS = ["afq_device.hpp", "afq_internal.hpp"]
+CPP_TESTS = [("tests/cpp/api_parity.cpp", "tests/cpp/api_parity")]  # host programs over the C++ mirror
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(ROOT, "include")]
@@ -59,6 +61,12 @@ def build(force=False, verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(LIB + ".tmp", LIB)
+    for src, exe in CPP_TESTS:
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, src),
+               "-o", os.path.join(ROOT, exe), "-L" + HERE, "-lvmc_afqmc", "-Wl,-rpath,$ORIGIN/../../vmc_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed for %s:\n%s" % (src, r.stderr))
     return LIB
 
 
