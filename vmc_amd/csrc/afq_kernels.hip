@@ -631,7 +631,207 @@ __global__ __launch_bounds__(256) void k_qr(double* psi, double* Qs, double* log
   }
 }
 
+// Register-resident variant of the same factorisation for N <= (512/CG)*MR,
+// n <= CG*MC.  512 threads per walker; thread (rg = tid/CG, cg = tid%CG) keeps rows
+// rg + (512/CG)m (m < MR) and columns cg + CG c (c < MC) of the determinant in
+// registers.  Per reflector: the column norm is a block reduction, the reflector
+// goes through LDS, w_j = sum_i conj(v_i) A_ij is reduced over the 64/CG lanes of
+// a wave that share a column group (lane shuffles) and then over the 8 waves
+// (LDS), and the rank-1 update is local.  The reflectors are parked in Vs (the walker's
+// scratch block) and applied backwards to [I; 0] the same way to form the thin Q.
+template <int QR_T, int CG, int MR, int MC>
+__global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double* logO, int N, int ndets, int na,
+                                                 int nb, int64_t offb, int64_t ms, int rhf_walker) {
+  constexpr int RG = QR_T / CG, QR_WAVES = QR_T / 64;
+  __shared__ cplx vl[RG * MR];
+  __shared__ cplx wsum[QR_WAVES][CG * MC];
+  __shared__ cplx wt[CG * MC];
+  __shared__ cplx tau[CG * MC];
+  __shared__ double red[16];
+  __shared__ double sbeta;
+  __shared__ cplx sden, sc0;
+  __shared__ int szero;
+  const int tid = threadIdx.x, cg = tid % CG, rg = tid / CG, lane = tid & 63, wave = tid >> 6;
+  const int w = blockIdx.x;
+  double lf_re = 0.0, lf_im = 0.0;
+  for (int d = 0; d < ndets; ++d) {
+    const int n = d == 0 ? na : nb;
+    cplx* A = reinterpret_cast<cplx*>(psi + 2 * ((int64_t)w * ms + (d == 0 ? 0 : offb)));
+    cplx* V = reinterpret_cast<cplx*>(Vs + 2 * ((int64_t)w * ms + (d == 0 ? 0 : offb)));
+    cplx a[MR][MC];
+#pragma unroll
+    for (int m = 0; m < MR; ++m)
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        const int i = rg + RG * m, j = cg + CG * c;
+        a[m][c] = (i < N && j < n) ? A[(int64_t)i * n + j] : cplx{0.0, 0.0};
+      }
+    // w_j (j > k, or j >= k when forming Q) of the reflector in vl, times t, into wt
+    auto reduce_w = [&](int k, int jmin, cplx t) {
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        const int j = cg + CG * c;
+        double sr = 0.0, si = 0.0;
+        if (j >= jmin && j < n) {
+#pragma unroll
+          for (int m = 0; m < MR; ++m) {
+            const int i = rg + RG * m;
+            if (i >= k && i < N) {
+              const cplx v = vl[i], x = a[m][c];
+              sr += v.re * x.re + v.im * x.im;
+              si += v.re * x.im - v.im * x.re;
+            }
+          }
+        }
+#pragma unroll
+        for (int off = CG; off < 64; off <<= 1) {
+          sr += __shfl_xor(sr, off, 64);
+          si += __shfl_xor(si, off, 64);
+        }
+        if (lane < CG) wsum[wave][j] = cplx{sr, si};
+      }
+      __syncthreads();
+      if (tid >= jmin && tid < n) {
+        cplx sum{0.0, 0.0};
+        for (int q = 0; q < QR_WAVES; ++q) sum = cadd(sum, wsum[q][tid]);
+        wt[tid] = cmul(t, sum);
+      }
+      __syncthreads();
+    };
+    // A_ij -= v_i wt_j for i >= k, j >= jmin
+    auto update = [&](int k, int jmin) {
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        const int j = cg + CG * c;
+        if (j >= jmin && j < n) {
+          const cplx u = wt[j];
+#pragma unroll
+          for (int m = 0; m < MR; ++m) {
+            const int i = rg + RG * m;
+            if (i >= k && i < N) a[m][c] = csub(a[m][c], cmul(vl[i], u));
+          }
+        }
+      }
+      __syncthreads();
+    };
+    for (int k = 0; k < n; ++k) {
+      const int kc = k % CG, ck = k / CG;
+      const bool own = cg == kc;
+      // column k of this thread's rows (selects, so the register tile keeps static indices)
+      cplx col[MR];
+#pragma unroll
+      for (int m = 0; m < MR; ++m) {
+        col[m] = a[m][0];
+#pragma unroll
+        for (int c = 1; c < MC; ++c)
+          if (c == ck) col[m] = a[m][c];
+      }
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < MR; ++m) {
+        const int i = rg + RG * m;
+        if (own && i > k && i < N) t += col[m].re * col[m].re + col[m].im * col[m].im;
+        if (own && i == k) sc0 = col[m];
+      }
+      const double tail = block_sum(t, red);
+      if (tid == 0) {
+        const cplx c0 = sc0;
+        if (tail <= DBL_MIN && c0.im * c0.im <= DBL_MIN) {
+          tau[k] = cplx{0.0, 0.0};
+          sbeta = c0.re;
+          szero = 1;
+        } else {
+          double beta = sqrt(c0.re * c0.re + c0.im * c0.im + tail);
+          if (c0.re >= 0.0) beta = -beta;
+          sbeta = beta;
+          sden = cplx{c0.re - beta, c0.im};
+          const cplx tt = cdiv(cplx{beta - c0.re, -c0.im}, cplx{beta, 0.0});
+          tau[k] = cplx{tt.re, -tt.im};
+          szero = 0;
+        }
+        lf_re += log(fabs(sbeta));
+        if (sbeta < 0.0) lf_im += M_PI;
+      }
+      __syncthreads();
+      const double beta = sbeta;
+      const int zero = szero;
+      const cplx den = sden;
+      if (own) {
+#pragma unroll
+        for (int m = 0; m < MR; ++m) {
+          const int i = rg + RG * m;
+          if (i > k && i < N) {
+            col[m] = zero ? cplx{0.0, 0.0} : cdiv(col[m], den);
+            vl[i] = col[m];
+          } else if (i == k) {
+            col[m] = cplx{beta, 0.0};
+            vl[i] = cplx{1.0, 0.0};
+          }
+#pragma unroll
+          for (int c = 0; c < MC; ++c)
+            if (c == ck) a[m][c] = col[m];
+        }
+      }
+      __syncthreads();
+      reduce_w(k, k + 1, tau[k]);
+      update(k, k + 1);
+    }
+    // park the reflectors, then Q = H_0^* ... H_{n-1}^* [I; 0],  H_k^* = I - conj(tau_k) v v^*
+#pragma unroll
+    for (int m = 0; m < MR; ++m)
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        const int i = rg + RG * m, j = cg + CG * c;
+        if (i < N && j < n) V[(int64_t)i * n + j] = a[m][c];
+        a[m][c] = cplx{i == j ? 1.0 : 0.0, 0.0};
+      }
+    __syncthreads();
+    for (int k = n - 1; k >= 0; --k) {
+      for (int i = tid; i < N; i += QR_T)
+        vl[i] = i > k ? V[(int64_t)i * n + k] : cplx{i == k ? 1.0 : 0.0, 0.0};
+      __syncthreads();
+      reduce_w(k, k, cplx{tau[k].re, -tau[k].im});
+      update(k, k);
+    }
+#pragma unroll
+    for (int m = 0; m < MR; ++m)
+#pragma unroll
+      for (int c = 0; c < MC; ++c) {
+        const int i = rg + RG * m, j = cg + CG * c;
+        if (i < N && j < n) A[(int64_t)i * n + j] = a[m][c];
+      }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double m = rhf_walker ? 2.0 : 1.0;
+    logO[2 * w] -= m * lf_re;
+    logO[2 * w + 1] -= m * lf_im;
+  }
+}
+
+template <int T, int CG, int MR, int MC>
+static void launch_qr_reg(const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL((k_qr_reg<T, CG, MR, MC>), dim3(s.W), dim3(T), 0, st, s.psi, s.t0, s.logO, s.N, s.ndets_prop,
+                     s.det_n[0], s.det_n[1], s.det_off[1], s.ms, (int)s.walker_rhf);
+}
+
+// k_qr_reg for N <= 320 and n <= 64 (4 column groups up to N = 256, whose tiles fit
+// the 256 registers of a 512-thread workgroup; 8 column groups with 5 row blocks
+// up to N = 320, which spill a little); k_qr otherwise.
 void launch_qr(const DevState& s, hipStream_t st) {
+  const int nmax = std::max(s.det_n[0], s.ndets_prop > 1 ? s.det_n[1] : 0);
+  if (s.N <= 256 && nmax <= 64) {
+    const bool m1 = s.N <= 128;
+    if (nmax <= 16) { if (m1) launch_qr_reg<512, 4, 1, 4>(s, st); else launch_qr_reg<512, 4, 2, 4>(s, st); }
+    else if (nmax <= 32) { if (m1) launch_qr_reg<512, 4, 1, 8>(s, st); else launch_qr_reg<512, 4, 2, 8>(s, st); }
+    else { if (m1) launch_qr_reg<512, 4, 1, 16>(s, st); else launch_qr_reg<512, 4, 2, 16>(s, st); }
+    return;
+  }
+  if (s.N <= 320 && nmax <= 64) {
+    if (nmax <= 32) launch_qr_reg<512, 8, 5, 4>(s, st);
+    else launch_qr_reg<512, 8, 5, 8>(s, st);
+    return;
+  }
   hipLaunchKernelGGL(k_qr, dim3(s.W), dim3(256), 0, st, s.psi, s.t0, s.logO, s.N, s.ndets_prop,
                      s.det_n[0], s.det_n[1], s.det_off[1], s.ms, (int)s.walker_rhf);
 }
