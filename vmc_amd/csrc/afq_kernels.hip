@@ -249,12 +249,15 @@ struct TB {  // [[B_re, B_im], [-B_im, B_re]] over the walker's propagated dets,
     return comp ? row[0] : -row[1];
   }
 };
-constexpr int VM_BM = 64, VM_BN = 128;
-__global__ __launch_bounds__(256) void k_vmul(const double* P, const double* tsrc, double* tdst,
+// One walker's V t for a BM x BN tile of (orbital row, determinant column) through
+// the generic engine; used when a walker has at most 64 electrons (k_taylor below
+// serves the wider determinants).
+template <int VM_BM, int VM_BN, int WVM, int WVN>
+__global__ __launch_bounds__(64 * WVM * WVN) void k_vmul(const double* P, const double* tsrc, double* tdst,
                                               double* acc, const double* weights, int N, int64_t T,
                                               int n2a, int n2b, int64_t offb, int64_t wstride,
                                               double sq, double inv_p, int gridx, int gridy, int W) {
-  using Eng = MmaF64<VM_BM, VM_BN, 2, 2, false, true>;
+  using Eng = MmaF64<VM_BM, VM_BN, WVM, WVN, false, true>;
   __shared__ double lds[Eng::LDS_DOUBLES];
   const TileId tl = tile_id(gridx, gridy, W, gridy);
   const int w = tl.z;
@@ -278,6 +281,119 @@ __global__ __launch_bounds__(256) void k_vmul(const double* P, const double* tsr
   });
 }
 
+// One Taylor product t_dst = (i sq P) t_src / p (acc += t_dst) for a 128-row x
+// 128-complex-column tile of one walker, 8 waves (2 x 4) on FP64 MFMA.  The complex
+// product is the real GEMM [A_re | A_im] [[T_re, T_im], [-T_im, T_re]] with
+// A = i P (A_re = -P_im, A_im = P_re); each K slice covers 8 orbitals, i.e. 16
+// real K rows: the P rows arrive as 16-byte pairs along k, the t rows as complex
+// numbers, and the embedding (swap and sign) is applied while staging into LDS.
+// The next slice is prefetched into registers while the current one multiplies.
+constexpr int TY_BM = 128, TY_BNC = 128, TY_BN = 2 * TY_BNC, TY_KC = 8, TY_K = 2 * TY_KC;
+constexpr int TY_LDA = TY_BM + 2, TY_LDB = TY_BN + 2;
+template <bool PAIRS>
+__global__ __launch_bounds__(512) void k_taylor(const double* P, const double* tsrc, double* tdst, double* acc,
+                                                const double* weights, int N, int na, int nb, int64_t offb,
+                                                int64_t wstride, double scale, int gridx, int gridy, int W) {
+  __shared__ double As[TY_K * TY_LDA];
+  __shared__ double Bs[TY_K * TY_LDB];
+  const TileId tl = tile_id(gridx, gridy, W, gridy);
+  const int w = tl.z;
+  if (weights[w] <= 1e-8) return;
+  const int row0 = tl.y * TY_BM, q0 = tl.x * TY_BNC, ncolc = na + nb;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 2, wn = wave & 3;
+  const double* Pre = P + (int64_t)(2 * w) * N * N;
+  const double* Pim = Pre + (int64_t)N * N;
+  const double* src = tsrc + (int64_t)w * wstride;
+  // A: thread -> (row r = tid / 4, orbital pair kp = 2 (tid % 4)) of the slice
+  const int ar = tid >> 2, akp = (tid & 3) * 2, ai = row0 + ar;
+  // B: thread -> (orbital kk, complex column qc) for two elements e = tid, tid + 512
+  double2 pa_re, pa_im, tb[2];
+  auto fetch = [&](int k0) {
+    const int k = k0 + akp;
+    pa_re = double2{0.0, 0.0};
+    pa_im = double2{0.0, 0.0};
+    if (ai < N) {
+      const int64_t o = (int64_t)ai * N + k;
+      if (PAIRS && k + 1 < N) {
+        pa_re = *reinterpret_cast<const double2*>(Pre + o);
+        pa_im = *reinterpret_cast<const double2*>(Pim + o);
+      } else {
+        if (k < N) { pa_re.x = Pre[o]; pa_im.x = Pim[o]; }
+        if (k + 1 < N) { pa_re.y = Pre[o + 1]; pa_im.y = Pim[o + 1]; }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = tid + 512 * h, kk = e / TY_BNC, qc = e % TY_BNC;
+      const int kr = k0 + kk, q = q0 + qc;
+      tb[h] = double2{0.0, 0.0};
+      if (kr < N && q < ncolc) {
+        const double* rowp = q < na ? src + 2 * ((int64_t)kr * na + q) : src + 2 * (offb + (int64_t)kr * nb + (q - na));
+        tb[h] = *reinterpret_cast<const double2*>(rowp);
+      }
+    }
+  };
+  auto stage = [&]() {
+    // A_re = -P_im (rows 0..7 of the slice), A_im = P_re (rows 8..15)
+    As[(akp + 0) * TY_LDA + ar] = -pa_im.x;
+    As[(akp + 1) * TY_LDA + ar] = -pa_im.y;
+    As[(TY_KC + akp + 0) * TY_LDA + ar] = pa_re.x;
+    As[(TY_KC + akp + 1) * TY_LDA + ar] = pa_re.y;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = tid + 512 * h, kk = e / TY_BNC, qc = e % TY_BNC;
+      Bs[kk * TY_LDB + 2 * qc] = tb[h].x;
+      Bs[kk * TY_LDB + 2 * qc + 1] = tb[h].y;
+      Bs[(TY_KC + kk) * TY_LDB + 2 * qc] = -tb[h].y;
+      Bs[(TY_KC + kk) * TY_LDB + 2 * qc + 1] = tb[h].x;
+    }
+  };
+  f64x4 c[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const int nst = (N + TY_KC - 1) / TY_KC;
+  fetch(0);
+  for (int st = 0; st < nst; ++st) {
+    stage();
+    __syncthreads();
+    if (st + 1 < nst) fetch((st + 1) * TY_KC);
+#pragma unroll
+    for (int k4 = 0; k4 < TY_K / 4; ++k4) {
+      const int kr = k4 * 4 + (lane >> 4);
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kr * TY_LDA + wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kr * TY_LDB + wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], c[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double* dst = tdst + (int64_t)w * wstride;
+  double* ac = acc + (int64_t)w * wstride;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+        const int cr = wn * 64 + j * 16 + (lane & 15);  // real column in the tile
+        const int q = q0 + (cr >> 1), comp = cr & 1;
+        if (row < N && q < ncolc) {
+          const int64_t a = (q < na ? 2 * ((int64_t)row * na + q) : 2 * (offb + (int64_t)row * nb + (q - na))) + comp;
+          const double x = c[i][j][r] * scale;
+          dst[a] = x;
+          ac[a] += x;
+        }
+      }
+}
+
 // psi <- E (sum_{k<=5} V^k / k!) E psi  for every live walker
 void launch_expmul_pre(const DevState& s, hipStream_t st) {
   const int64_t ws = 2 * s.ms;
@@ -296,10 +412,22 @@ void launch_taylor(const DevState& s, hipStream_t st) {
   const double sq = sqrt(s.dt);
   double* src = s.t0;
   double* dst = s.t1;
+  const bool wide = n2a + n2b > 128;
+  const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
   for (int p = 1; p <= 5; ++p) {
-    const int gx = (int)cdivu(n2a + n2b, VM_BN), gy = (int)cdivu(s.N, VM_BM);
-    hipLaunchKernelGGL(k_vmul, dim3(gx * gy * s.W), dim3(256), 0, st, s.P, src, dst, s.accb, s.weights, s.N,
-                       s.T, n2a, n2b, offb, ws, sq, 1.0 / p, gx, gy, s.W);
+    if (wide) {
+      const int gx = (int)cdivu(na + nb, TY_BNC), gy = (int)cdivu(s.N, TY_BM);
+      if (s.N % 2 == 0)
+        hipLaunchKernelGGL(k_taylor<true>, dim3(gx * gy * s.W), dim3(512), 0, st, s.P, src, dst, s.accb, s.weights,
+                           s.N, na, nb, s.det_off[1], ws, sq / p, gx, gy, s.W);
+      else
+        hipLaunchKernelGGL(k_taylor<false>, dim3(gx * gy * s.W), dim3(512), 0, st, s.P, src, dst, s.accb, s.weights,
+                           s.N, na, nb, s.det_off[1], ws, sq / p, gx, gy, s.W);
+    } else {
+      const int gx = (int)cdivu(n2a + n2b, 128), gy = (int)cdivu(s.N, 64);
+      hipLaunchKernelGGL((k_vmul<64, 128, 2, 2>), dim3(gx * gy * s.W), dim3(256), 0, st, s.P, src, dst, s.accb,
+                         s.weights, s.N, s.T, n2a, n2b, offb, ws, sq, 1.0 / p, gx, gy, s.W);
+    }
     double* t = src; src = dst; dst = t;
   }
 }
