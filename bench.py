@@ -8,9 +8,10 @@ and the local energy of every walker.  Inputs are resident in HBM before the
 timed region.  Paths:
 
   fp64               exact FP64 local energy of every walker;
-  mixed_exchange_cv  control variate: the exchange in one BF16 MFMA pass on every
-                     walker, exact FP64 on a random 1/subset_inv of them
-                     (Horvitz-Thompson correction, unbiased); the headline value;
+  mixed_exchange_cv  force bias and VHS in split BF16 (three passes, FP32
+                     accumulation); control variate: the exchange in one BF16 MFMA
+                     pass on every live walker, exact FP64 on a random 1/subset_inv
+                     of them (Horvitz-Thompson correction, unbiased); the headline;
   mlp_cv             control variate with the learned MLP surrogate, trained on
                      the GPU from the first exact steps.
 
@@ -124,8 +125,13 @@ def stage_rooflines(spec, path, n_exact):
             exchange_flops(spec, W), 1, "bf16", walkers=W, passes=passes)
     add("taylor", "k_vmul (5 complex V t products, FP64 MFMA)", W * sum(40.0 * N * N * n for n in nsp), 5, "fp64")
     add("onebody", "k_expmul (E psi, FP64 MFMA)", W * sum(8.0 * N * N * n for n in nsp), 2 * len(nsp), "fp64")
-    add("force_bias", "k_force_bias (FP64 MFMA)", W * sum(4.0 * n * N * G for n in nsp), 1, "fp64")
-    add("vhs", "k_vhs (FP64 MFMA)", W * 4.0 * T * G, 1, "fp64")
+    if path.get("fb_vhs_prec", "fp64") == "fp64":
+        add("force_bias", "k_force_bias (FP64 MFMA)", W * sum(4.0 * n * N * G for n in nsp), 1, "fp64")
+        add("vhs", "k_vhs (FP64 MFMA)", W * 4.0 * T * G, 1, "fp64")
+    else:
+        add("force_bias", "k_theta_split + k_force_bias_bf16<2> (3-pass split BF16 MFMA)",
+            W * sum(4.0 * n * N * G for n in nsp), 1, "bf16", passes=3)
+        add("vhs", "k_fields_split + k_vhs_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * T * G, 1, "bf16", passes=3)
     add("green", "k_green (FP64 MFMA + Gauss-Jordan)", W * sum(4 * n * n * N + 16 * n ** 3 + 8 * N * n * n
                                                               for n in nsp), 1, "fp64")
     return out
@@ -186,6 +192,8 @@ def main():
                          "always measured exactly")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-surrogate path")
+    ap.add_argument("--fp64-propagation", action="store_true",
+                    help="mixed path: keep the force bias and VHS in FP64 (default: split BF16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
@@ -196,7 +204,8 @@ def main():
               "--gpus %d is not supported" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
-    from vmc_amd.abi import AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16, AFQ_PREC_FP64, AFQ_SURR_EXCHANGE, AFQ_SURR_MLP
+    from vmc_amd.abi import (AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64, AFQ_SURR_EXCHANGE,
+                             AFQ_SURR_MLP)
     from vmc_amd.synthetic import workload
     t0 = time.time()
     ham, spec = workload(a.workload)
@@ -217,11 +226,13 @@ def main():
     if not a.fp64_only:
         # surrogate = the exchange in one BF16 pass on every walker; exact FP64 on a
         # random 1/subset_inv of the walkers (Horvitz-Thompson correction)
+        pp = AFQ_PREC_FP64 if a.fp64_propagation else AFQ_PREC_BF16X3
         xcv = run_path(ham, spec, a.warmup + a.steps, a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
-                       prec_exchange=AFQ_PREC_FP64, surrogate_kind=AFQ_SURR_EXCHANGE,
+                       prec_fb=pp, prec_vhs=pp, prec_exchange=AFQ_PREC_FP64, surrogate_kind=AFQ_SURR_EXCHANGE,
                        surrogate_prec=AFQ_PREC_BF16, subset_inv=a.subset_inv, risk_frac=0.0, shadow_exact=True)
         nx = (W + a.subset_inv - 1) // a.subset_inv
-        xcv.update(exchange_prec="fp64", surrogate_prec="bf16", n_exact=nx)
+        xcv.update(exchange_prec="fp64", surrogate_prec="bf16", n_exact=nx,
+                   fb_vhs_prec="fp64" if a.fp64_propagation else "bf16x3")
         xcv.update(cv_summary(xcv, fp64))
         paths["mixed_exchange_cv"] = xcv
         log("[bench] mixed (bf16 exchange level + fp64 1/%d subset): %.1f walker-steps/s, %.1f ms/step, "

@@ -123,8 +123,11 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
 int afq_destroy(afq_ctx* ctx);
 
 /* Change the arithmetic of the stages on an existing context (the walker
- * population is kept).  This build supports prec_fb = prec_vhs = AFQ_PREC_FP64
- * and prec_exchange in {FP64, BF16X3, BF16}. */
+ * population is kept).  Each of prec_fb (force bias), prec_vhs (VHS) and
+ * prec_exchange (exchange contraction) is AFQ_PREC_FP64, AFQ_PREC_BF16X3 (bf16
+ * hi/lo split, three MFMA passes, FP32 accumulation: about FP32 accuracy) or
+ * AFQ_PREC_BF16 (one pass).  Overlaps, Green's functions, the propagation,
+ * weights and QR stay FP64.  ref_compat contexts need prec_vhs = AFQ_PREC_FP64. */
 int afq_set_precision(afq_ctx* ctx, int32_t prec_fb, int32_t prec_vhs, int32_t prec_exchange);
 
 /* Number of Cholesky vectors used by the local energy (default nchol). */

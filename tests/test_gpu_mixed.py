@@ -32,3 +32,52 @@ def test_exchange_bf16x3_vs_fp64(name):
     print(name, "max |dE| bf16x3 %.2e  bf16 %.2e" % (np.max(np.abs(e3 - e64)), np.max(np.abs(e1 - e64))))
     assert np.max(np.abs(e3 - e64) / np.abs(e64)) < 5e-6
     assert np.max(np.abs(e1 - e64)) < 0.5
+
+
+@pytest.mark.parametrize("name", ["uhf_uhf", "rhf_rhf", "c0_rhf", "uhf_33_31", "uhf_n60", "c1_small"])
+def test_force_bias_bf16x3_vs_fp64(name):
+    """Split-BF16 force bias against FP64 on the same walkers (relative to the
+    largest |fb| of each walker)."""
+    W = 6
+    h, wrhf, ctx, orc = build(name, W)
+    dets = perturbed_walkers(h, W, wrhf, np.random.default_rng(12), 0.05)
+    ctx.set_walkers(dets)
+    f64 = ctx.force_bias()
+    ctx.set_precision(prec_fb=AFQ_PREC_BF16X3)
+    ctx.set_walkers(dets)
+    f3 = ctx.force_bias()
+    err = np.max(np.abs(f3 - f64), axis=1) / np.max(np.abs(f64), axis=1)
+    print(name, "fb bf16x3 rel err %.2e" % err.max())
+    assert err.max() < 2e-5
+
+
+@pytest.mark.parametrize("name", ["uhf_uhf", "c0_rhf", "uhf_n60", "c1_small"])
+def test_mixed_step_vs_fp64(name):
+    """Three steps from the same walkers with the same draws: force bias, VHS and
+    exchange in BF16x3 against all-FP64.  The propagated walkers, weights and
+    local energies stay within 1e-5 (relative) of the FP64 path."""
+    W = 6
+    h, wrhf, ctx, orc = build(name, W)
+    h2, _, ctx2, _ = build(name, W)
+    rng = np.random.default_rng(13)
+    dets = perturbed_walkers(h, W, wrhf, rng, 0.05)
+    o = ctx.ham_and_overlap(h.serial_det(*ctx.ref_dets)[None, :], h.nchol)[0]
+    eshift = (o[0] / o[1]).real
+    ctx.set_walkers(dets)
+    ctx2.set_precision(prec_fb=AFQ_PREC_BF16X3, prec_vhs=AFQ_PREC_BF16X3, prec_exchange=AFQ_PREC_BF16X3)
+    ctx2.set_walkers(dets)
+    for step in range(3):
+        x = rng.standard_normal((W, h.nchol))
+        ctx.propagate(x, eshift)
+        ctx2.propagate(x, eshift)
+    d1, o1, w1 = ctx.get_walkers()
+    d2, o2, w2 = ctx2.get_walkers()
+    live = w1 > 0
+    assert np.array_equal(live, w2 > 0)
+    e1, _ = ctx.measure()
+    e2, _ = ctx2.measure()
+    rd = np.max(np.abs(d2 - d1)) / np.max(np.abs(d1))
+    rw = np.max(np.abs(w2[live] - w1[live]) / w1[live])
+    re = np.max(np.abs(e2[live] - e1[live]) / np.abs(e1[live]))
+    print(name, "mixed vs fp64 after 3 steps: dets %.2e weights %.2e E_L %.2e" % (rd, rw, re))
+    assert rd < 1e-5 and rw < 1e-5 and re < 1e-5

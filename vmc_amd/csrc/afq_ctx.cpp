@@ -179,7 +179,7 @@ void free_all(DevState& s) {
     if (p) (void)hipFree(p);
   if (s.idx) (void)hipFree(s.idx);
   if (s.ncap) (void)hipFree(s.ncap);
-  unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo};
+  unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo, s.Lt_hi, s.Lt_lo, s.Fb_hi, s.Fb_lo};
   for (unsigned short* p : hp)
     if (p) (void)hipFree(p);
 }
@@ -197,6 +197,24 @@ void ensure_lowprec(DevState& s, hipStream_t st) {
   launch_check("rsplit");
 }
 
+// bf16 operand copies for the low-precision VHS (allocated on first use)
+void ensure_vhs_lowprec(DevState& s, hipStream_t st) {
+  if (s.Lt_hi) return;
+  s.Gp = (s.G + 7) / 8 * 8;
+  s.Lt_hi = dalloc<unsigned short>(s.T * s.Gp);
+  s.Lt_lo = dalloc<unsigned short>(s.T * s.Gp);
+  s.Fb_hi = dalloc<unsigned short>((int64_t)s.W * 2 * s.Gp);
+  s.Fb_lo = dalloc<unsigned short>((int64_t)s.W * 2 * s.Gp);
+  afq::launch_lt_split(s, st);
+  launch_check("lt_split");
+}
+
+// force bias of the current theta in the context's force-bias precision
+void run_force_bias(const DevState& s, hipStream_t st) {
+  if (s.prec_fb == AFQ_PREC_FP64) afq::launch_force_bias(s, st);
+  else afq::launch_force_bias_lp(s, s.prec_fb, st);
+}
+
 bool exchange_prec_ok(int p) {
   return p == AFQ_PREC_FP64 || p == AFQ_PREC_BF16X3 || p == AFQ_PREC_BF16;
 }
@@ -209,6 +227,7 @@ void free_temporary(const DevState& t) {
   z.E = z.Lp = z.R = z.mf = nullptr;
   z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
   z.Rb_hi = z.Rb_lo = z.Tb_hi = z.Tb_lo = nullptr;
+  z.Lt_hi = z.Lt_lo = z.Fb_hi = z.Fb_lo = nullptr;
   free_all(z);
 }
 
@@ -339,7 +358,7 @@ void refresh_walkers(afq_ctx* c, bool overlaps_from_dets) {
     launch_check("logO");
   }
   Stage t(c, afq::ST_FB);
-  afq::launch_force_bias(s, c->stream);
+  run_force_bias(s, c->stream);
   launch_check("force_bias");
 }
 
@@ -439,9 +458,11 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
   if (nb == 0) return fail(nullptr, AFQ_ERR_UNSUPPORTED, "nbeta == 0 not supported");
   if (na > 64 || nb > 64)
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "more than 64 electrons per spin not supported by this build");
-  if (params->prec_fb != AFQ_PREC_FP64 || params->prec_vhs != AFQ_PREC_FP64 ||
+  if (!exchange_prec_ok(params->prec_fb) || !exchange_prec_ok(params->prec_vhs) ||
       !exchange_prec_ok(params->prec_exchange))
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
+  if (params->ref_compat && params->prec_vhs != AFQ_PREC_FP64)
+    return fail(nullptr, AFQ_ERR_UNSUPPORTED, "ref_compat rounds the VHS itself; it needs prec_vhs = FP64");
   afq_ctx* c = new afq_ctx();
   try {
     c->params = *params;
@@ -474,6 +495,8 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     s.ctot = 0;
     for (int q = 0; q < s.nslots; ++q) s.ctot += 2 * s.slot[q].n;
     s.prec_exchange = params->prec_exchange;
+    s.prec_fb = params->prec_fb;
+    s.prec_vhs = params->prec_vhs;
     int ngt = 0;
     for (int q = 0; q < s.nslots; ++q)
       ngt = std::max(ngt, (G + afq::exchange_tile_g(s.slot[q].n) - 1) / afq::exchange_tile_g(s.slot[q].n));
@@ -516,7 +539,8 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     s.mf = dalloc<double>(2 * (size_t)G);
     alloc_walkers(s);
     afq::launch_fill(s.weights, W, 0.0, c->stream);
-    if (s.prec_exchange != AFQ_PREC_FP64) ensure_lowprec(s, c->stream);
+    if (s.prec_exchange != AFQ_PREC_FP64 || s.prec_fb != AFQ_PREC_FP64) ensure_lowprec(s, c->stream);
+    if (s.prec_vhs != AFQ_PREC_FP64) ensure_vhs_lowprec(s, c->stream);
     hip_check(hipStreamSynchronize(c->stream), "setup sync");
   } catch (const DevError& e) {
     free_all(c->s);
@@ -546,12 +570,19 @@ int afq_destroy(afq_ctx* c) {
 
 int afq_set_precision(afq_ctx* c, int32_t prec_fb, int32_t prec_vhs, int32_t prec_exchange) {
   if (!c) return AFQ_ERR_ARG;
-  if (prec_fb != AFQ_PREC_FP64 || prec_vhs != AFQ_PREC_FP64 || !exchange_prec_ok(prec_exchange))
+  if (!exchange_prec_ok(prec_fb) || !exchange_prec_ok(prec_vhs) || !exchange_prec_ok(prec_exchange))
     return fail(c, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
+  if (c->s.ref_compat && prec_vhs != AFQ_PREC_FP64)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "ref_compat rounds the VHS itself; it needs prec_vhs = FP64");
   AFQ_GUARD_BEGIN
-  if (prec_exchange != AFQ_PREC_FP64) ensure_lowprec(c->s, c->stream);
+  if (prec_exchange != AFQ_PREC_FP64 || prec_fb != AFQ_PREC_FP64) ensure_lowprec(c->s, c->stream);
+  if (prec_vhs != AFQ_PREC_FP64) ensure_vhs_lowprec(c->s, c->stream);
   c->s.prec_exchange = prec_exchange;
+  c->s.prec_fb = prec_fb;
+  c->s.prec_vhs = prec_vhs;
   c->params.prec_exchange = prec_exchange;
+  c->params.prec_fb = prec_fb;
+  c->params.prec_vhs = prec_vhs;
   return AFQ_OK;
   AFQ_GUARD_END(c)
 }
@@ -670,7 +701,8 @@ int afq_propagate(afq_ctx* c, const double* draws, uint64_t seed, uint64_t step,
   }
   {
     Stage t(c, afq::ST_VHS);
-    afq::launch_vhs(s, c->stream);
+    if (s.prec_vhs == AFQ_PREC_FP64) afq::launch_vhs(s, c->stream);
+    else afq::launch_vhs_lp(s, s.prec_vhs, c->stream);
     launch_check("vhs");
   }
   {
@@ -698,7 +730,7 @@ int afq_propagate(afq_ctx* c, const double* draws, uint64_t seed, uint64_t step,
   }
   {
     Stage t(c, afq::ST_FB);
-    afq::launch_force_bias(s, c->stream);
+    run_force_bias(s, c->stream);
     launch_check("force_bias");
   }
   if (factors) download(factors, s.factors, s.W, c->stream);
@@ -898,7 +930,7 @@ int afq_ham_and_overlap(afq_ctx* c, const double* dets, int32_t ndet, int32_t nc
     afq::launch_fill(t.weights, ndet, 1.0, c->stream);
     afq::launch_green(t, true, c->stream);
     afq::launch_logO_from_logdet(t, c->stream);
-    afq::launch_force_bias(t, c->stream);
+    afq::launch_force_bias(t, c->stream);  // FP64: this population is not the one the bf16 buffers cover
     if (t.GE > 0) afq::launch_exchange(t, nullptr, ndet, c->stream);
     afq::launch_energy(t, nullptr, ndet, 0.0, c->stream);
     launch_check("ham_and_overlap");

@@ -80,7 +80,11 @@ struct DevState {
   int Np = 0;               // N padded to a multiple of 8 (bf16 rows)
   int ctot = 0;             // theta columns per walker, 2 sum_s n_s
   unsigned short* Rb_hi = nullptr; unsigned short* Rb_lo = nullptr;  // [slot][g][i][Np]
-  unsigned short* Tb_hi = nullptr; unsigned short* Tb_lo = nullptr;  // [w][ctot][Np]
+  unsigned short* Tb_hi = nullptr; unsigned short* Tb_lo = nullptr;  // [w][re/im][slot][j][Np]
+  int prec_fb = AFQ_PREC_FP64, prec_vhs = AFQ_PREC_FP64;
+  int Gp = 0;               // G padded to a multiple of 8
+  unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
+  unsigned short* Fb_hi = nullptr; unsigned short* Fb_lo = nullptr;  // fields, [w][re/im][Gp]
 };
 
 // Exchange-energy surrogate: GELU MLP dims[0] -> dims[1] -> dims[2] -> dims[3] -> 1,
@@ -143,6 +147,10 @@ void launch_zero_counter(unsigned long long* p, hipStream_t st);
 
 void launch_rsplit(const DevState& s, hipStream_t st);
 void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, hipStream_t st);
+void launch_theta_split(const DevState& s, const int* sel, int nsel, hipStream_t st);
+void launch_force_bias_lp(const DevState& s, int prec, hipStream_t st);
+void launch_lt_split(const DevState& s, hipStream_t st);
+void launch_vhs_lp(const DevState& s, int prec, hipStream_t st);
 
 int exchange_tile_g(int n);   // g-blocks per exchange tile for slot size n (0: unsupported)
 int exchange_tile_w(int n);   // walkers per exchange tile

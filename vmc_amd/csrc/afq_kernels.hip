@@ -1073,7 +1073,7 @@ void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st
 // ------------------------------------------------------------------ mixed-precision exchange
 // bf16 hi/lo copies, K (orbital) contiguous and padded to Np:
 //   Rb[c][slot][g][i][Np]          from R[g][koff_s + k*n_s + i]
-//   Tb[c][w][col][Np], col = cols_off_s + (j, re/im)   from theta_s[w][k][j][re/im]
+//   Tb[c][w][re/im][slot][j][Np]                   from theta_s[w][k][j][re/im]
 __global__ void k_rsplit(const double* R, u16* hi, u16* lo, int N, int Np, int G, int64_t Ktot, int64_t koff,
                          int n, int64_t dst_off) {
   const int64_t total = (int64_t)G * n * Np;
@@ -1098,6 +1098,9 @@ void launch_rsplit(const DevState& s, hipStream_t st) {
 }
 
 // transpose + split theta of the selected walkers (one workgroup per walker and slot)
+// into Tb[w][re/im][slot][j][k]: every (walker, component) row holds the slots'
+// columns back to back, so the force bias reads it as one K-contiguous row and the
+// exchange reads each (walker, component, slot, j) as a K-contiguous column
 __global__ __launch_bounds__(256) void k_theta_split(const double* theta, const int* sel, u16* hi, u16* lo, int N,
                                                      int Np, int64_t Ktot, Slot s0, Slot s1, int ctot) {
   __shared__ double tile[32 * 129];
@@ -1117,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_theta_split(const double* theta, const 
       if (r0 + r < Np) {
         u16 h, l;
         split_bf16(tile[r * 129 + c], h, l);
-        const int64_t a = ((int64_t)w * ctot + coff + c) * Np + r0 + r;
+        const int64_t a = (((int64_t)w * 2 + (c & 1)) * (ctot / 2) + coff / 2 + (c >> 1)) * Np + r0 + r;
         hi[a] = h;
         lo[a] = l;
       }
@@ -1159,7 +1162,7 @@ __global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u
         const int wl = q / (2 * n), cc = q - wl * 2 * n;
         if (wl >= wt || w0 + wl >= nw) return nullptr;
         const int w = sel ? sel[w0 + wl] : w0 + wl;
-        return (c ? Tl : Th) + ((int64_t)w * ctot + coff + cc) * Np;
+        return (c ? Tl : Th) + (((int64_t)w * 2 + (cc & 1)) * (ctot / 2) + coff / 2 + (cc >> 1)) * Np;
       },
       Np, lds);
   float* F = reinterpret_cast<float*>(lds);
@@ -1232,8 +1235,8 @@ void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, h
     bb.koff = s.slot[0].koff;
     const u16* rh = s.Rb_hi + (int64_t)s.G * s.slot[0].n * s.Np;
     const u16* rl = s.Rb_lo + (int64_t)s.G * s.slot[0].n * s.Np;
-    const u16* th = s.Tb_hi + (int64_t)2 * s.slot[0].n * s.Np;
-    const u16* tlo = s.Tb_lo + (int64_t)2 * s.slot[0].n * s.Np;
+    const u16* th = s.Tb_hi + (int64_t)s.slot[0].n * s.Np;
+    const u16* tlo = s.Tb_lo + (int64_t)s.slot[0].n * s.Np;
     double* xp = s.Xp + 2 * (int64_t)s.W * s.ngt_cap;
     if (prec == AFQ_PREC_BF16X3)
       hipLaunchKernelGGL(k_exchange_bf16<2>, dim3(ngt * gy), dim3(256), 0, st, rh, rl, th, tlo, sel, xp, s.Np, s.GE,
@@ -1242,6 +1245,154 @@ void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, h
       hipLaunchKernelGGL(k_exchange_bf16<1>, dim3(ngt * gy), dim3(256), 0, st, rh, rl, th, tlo, sel, xp, s.Np, s.GE,
                          nsel, s.W, s.G, bb, bb, s.ctot, s.ngt_cap, ngt, gy, 1);
   }
+}
+
+// ------------------------------------------------------------------ mixed-precision force bias
+// fb[w][g][c] = alpha sum_{s,i,k} Tb[w][c][s][i][k] Rb[s][g][i][k]: rows (w, c) of the
+// split theta against rows g of the split rotated Cholesky, one K-contiguous run of
+// n_s * Np per slot, FP32 accumulation.
+template <int NS>
+__global__ __launch_bounds__(256, 2) void k_force_bias_bf16(const u16* Rh, const u16* Rl, const u16* Th,
+                                                            const u16* Tl, double* fb, int G, int W, int Np,
+                                                            int nsum, int nslots, Slot s0, Slot s1, double alpha,
+                                                            int gridx, int gridy) {
+  using Eng = MmaBF16<128, 128, 2, 2, NS, NS>;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[Eng::LDS_BYTES];
+  const TileId tl = tile_id(gridx, gridy, 1, 8);
+  const int row0 = tl.y * 128, g0 = tl.x * 128;
+  Eng e;
+  int64_t roff = 0;
+  int coff = 0;
+  for (int z = 0; z < nslots; ++z) {
+    const int n = z == 0 ? s0.n : s1.n;
+    const int64_t ro = roff;
+    const int co = coff;
+    e.run(
+        [&](int c, int r) -> const u16* {
+          const int q = row0 + r;
+          if (q >= 2 * W) return nullptr;
+          return (c ? Tl : Th) + ((int64_t)q * nsum + co) * Np;
+        },
+        [&](int c, int col) -> const u16* {
+          const int g = g0 + col;
+          if (g >= G) return nullptr;
+          return (c ? Rl : Rh) + ro + (int64_t)g * n * Np;
+        },
+        n * Np, lds);
+    __syncthreads();
+    roff += (int64_t)G * n * Np;
+    coff += n;
+  }
+  e.store([&](int r, int c, float v) {
+    const int q = row0 + r, g = g0 + c;
+    if (q < 2 * W && g < G) fb[(int64_t)(q >> 1) * 2 * G + 2 * g + (q & 1)] = alpha * (double)v;
+  });
+}
+
+void launch_force_bias_lp(const DevState& s, int prec, hipStream_t st) {
+  launch_theta_split(s, nullptr, s.W, st);
+  const int gx = (int)cdivu(s.G, 128), gy = (int)cdivu(2 * (int64_t)s.W, 128);
+  const Slot& b = s.slot[s.nslots > 1 ? 1 : 0];
+  if (prec == AFQ_PREC_BF16X3)
+    hipLaunchKernelGGL(k_force_bias_bf16<2>, dim3(gx * gy), dim3(256), 0, st, s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo,
+                       s.fb, s.G, s.W, s.Np, s.ctot / 2, s.nslots, s.slot[0], b, (double)s.slot[0].mult, gx, gy);
+  else
+    hipLaunchKernelGGL(k_force_bias_bf16<1>, dim3(gx * gy), dim3(256), 0, st, s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo,
+                       s.fb, s.G, s.W, s.Np, s.ctot / 2, s.nslots, s.slot[0], b, (double)s.slot[0].mult, gx, gy);
+}
+
+// ------------------------------------------------------------------ mixed-precision VHS
+// Lt[t][g] = split(Lp[g][t]) (once per context), through a 32 x 32 LDS transpose.
+__global__ __launch_bounds__(256) void k_lt_split(const double* Lp, u16* hi, u16* lo, int64_t T, int G, int Gp) {
+  __shared__ double tile[32][33];
+  const int64_t t0 = (int64_t)blockIdx.x * 32;
+  const int g0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int g = g0 + r;
+    const int64_t t = t0 + tx;
+    tile[r][tx] = (g < G && t < T) ? Lp[(int64_t)g * T + t] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t t = t0 + r;
+    const int g = g0 + tx;
+    if (t < T && g < Gp) {
+      u16 h, l;
+      split_bf16(tile[tx][r], h, l);
+      hi[t * Gp + g] = h;
+      lo[t * Gp + g] = l;
+    }
+  }
+}
+void launch_lt_split(const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_lt_split, dim3((unsigned)cdivu(s.T, 32), (unsigned)cdivu(s.Gp, 32)), dim3(256), 0, st, s.Lp,
+                     s.Lt_hi, s.Lt_lo, s.T, s.G, s.Gp);
+}
+
+// Fb[w][c][g] = split(fields[w][g][c])
+__global__ void k_fields_split(const double* fields, u16* hi, u16* lo, int W, int G, int Gp) {
+  const int64_t total = (int64_t)W * 2 * Gp;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(e % Gp);
+    const int64_t q = e / Gp;
+    const double x = g < G ? fields[(q >> 1) * 2 * G + 2 * g + (q & 1)] : 0.0;
+    u16 h, l;
+    split_bf16(x, h, l);
+    hi[e] = h;
+    lo[e] = l;
+  }
+}
+
+// P[w][c] (both triangles) = sum_g field[w][g][c] L_g from the split operands
+template <int NS>
+__global__ __launch_bounds__(256, 2) void k_vhs_bf16(const u16* Fh, const u16* Fl, const u16* Lh, const u16* Ll,
+                                                     double* P, const double* weights, int W, int64_t T, int N,
+                                                     int Gp, int gridx, int gridy) {
+  using Eng = MmaBF16<128, 128, 2, 2, NS, NS>;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[Eng::LDS_BYTES];
+  const TileId tl = tile_id(gridx, gridy, 1, 8);
+  const int row0 = tl.y * 128;
+  const int64_t t0 = (int64_t)tl.x * 128;
+  bool any = false;
+  for (int q = row0; q < min(row0 + 128, 2 * W); q += 2) any |= weights[q >> 1] > 1e-8;
+  if (!any) return;
+  Eng e;
+  e.run(
+      [&](int c, int r) -> const u16* {
+        const int q = row0 + r;
+        return q < 2 * W ? (c ? Fl : Fh) + (int64_t)q * Gp : nullptr;
+      },
+      [&](int c, int col) -> const u16* {
+        const int64_t t = t0 + col;
+        return t < T ? (c ? Ll : Lh) + t * Gp : nullptr;
+      },
+      Gp, lds);
+  e.store([&](int r, int c, float v) {
+    const int q = row0 + r;
+    const int64_t t = t0 + c;
+    if (q >= 2 * W || t >= T) return;
+    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    if ((int64_t)(i + 1) * (i + 2) / 2 <= t) ++i;
+    if ((int64_t)i * (i + 1) / 2 > t) --i;
+    const int j = (int)(t - (int64_t)i * (i + 1) / 2);
+    double* Pq = P + (int64_t)q * N * N;
+    Pq[(int64_t)i * N + j] = (double)v;
+    Pq[(int64_t)j * N + i] = (double)v;
+  });
+}
+
+void launch_vhs_lp(const DevState& s, int prec, hipStream_t st) {
+  const int64_t tot = (int64_t)s.W * 2 * s.Gp;
+  hipLaunchKernelGGL(k_fields_split, dim3((unsigned)std::min<int64_t>(cdivu(tot, 256), 65535)), dim3(256), 0, st,
+                     s.fields, s.Fb_hi, s.Fb_lo, s.W, s.G, s.Gp);
+  const int gx = (int)cdivu(s.T, 128), gy = (int)cdivu(2 * (int64_t)s.W, 128);
+  if (prec == AFQ_PREC_BF16X3)
+    hipLaunchKernelGGL(k_vhs_bf16<2>, dim3(gx * gy), dim3(256), 0, st, s.Fb_hi, s.Fb_lo, s.Lt_hi, s.Lt_lo, s.P,
+                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy);
+  else
+    hipLaunchKernelGGL(k_vhs_bf16<1>, dim3(gx * gy), dim3(256), 0, st, s.Fb_hi, s.Fb_lo, s.Lt_hi, s.Lt_lo, s.P,
+                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy);
 }
 
 // E_L = ecore + sum_s m_s e1_s + 1/2 sum_{g<GE} fb_g^2 - 1/2 sum_s m_s X_s
