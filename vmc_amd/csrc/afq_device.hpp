@@ -247,19 +247,21 @@ __device__ __forceinline__ void split_bf16(double x, u16& hi, u16& lo) {
   lo = f2bf_bits((float)(x - (double)bf_bits2f(hi)));
 }
 
-template <int BM, int BN, int WAVES_M, int WAVES_N, int NA, int NB>
+template <int BM, int BN, int WAVES_M, int WAVES_N, int NA, int NB, int BK_ = 32>
 struct MmaBF16 {
-  static constexpr int BK = 32;
-  static constexpr int LDK = BK + 8;  // bf16 per LDS row (80 B, keeps 16-B alignment)
+  static constexpr int BK = BK_;
+  static constexpr int CPR = BK / 8;  // 16-byte chunks per row and stage
+  static constexpr int LDK = BK + 8;  // bf16 per LDS row (keeps 16-B alignment)
   static constexpr int NT = 64 * WAVES_M * WAVES_N;
   static constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   static constexpr int TM = WM / 16, TN = WN / 16;
-  static constexpr int A_CH = NA * BM * (BK / 8);  // 16-byte chunks per stage
-  static constexpr int B_CH = NB * BN * (BK / 8);
+  static constexpr int A_CH = NA * BM * CPR;  // 16-byte chunks per stage
+  static constexpr int B_CH = NB * BN * CPR;
   static constexpr int A_PER_T = (A_CH + NT - 1) / NT;
   static constexpr int B_PER_T = (B_CH + NT - 1) / NT;
   static constexpr int LDS_BYTES = (NA * BM + NB * BN) * LDK * 2;
   static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
+  static_assert(BK % 32 == 0, "stage depth must be a multiple of the MFMA K (32)");
 
   f32x4 acc[TM][TN];
   int lane, wm, wn;
@@ -276,7 +278,9 @@ struct MmaBF16 {
   }
 
   // pa(c, r) / pb(c, col): const u16* to the K-contiguous row of component c, or nullptr.
-  // Kp: padded K (multiple of 8); rows hold at least Kp elements.
+  // Kp: padded K (multiple of 8); rows hold at least Kp elements.  The loaders are
+  // evaluated once per tile: each thread keeps the row pointers of the chunks it
+  // moves and only advances k between stages.
   template <class PA, class PB>
   __device__ __forceinline__ void run(const PA& pa, const PB& pb, int Kp, void* ldsv) {
     u16* lds = reinterpret_cast<u16*>(ldsv);
@@ -284,39 +288,52 @@ struct MmaBF16 {
     u16* Bs = lds + NA * BM * LDK;    // [NB][BN][LDK]
     const int tid = threadIdx.x;
     uint4 ra[A_PER_T], rb[B_PER_T];
+    const u16* pA[A_PER_T];
+    const u16* pB[B_PER_T];
+#pragma unroll
+    for (int q = 0; q < A_PER_T; ++q) {
+      const int e = tid + q * NT;
+      pA[q] = nullptr;
+      if (e < A_CH) {
+        const int ch = e % CPR, r = (e / CPR) % BM, c = (e / CPR) / BM;
+        const u16* p = pa(c, r);
+        if (p) pA[q] = p + ch * 8;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER_T; ++q) {
+      const int e = tid + q * NT;
+      pB[q] = nullptr;
+      if (e < B_CH) {
+        const int ch = e % CPR, r = (e / CPR) % BN, c = (e / CPR) / BN;
+        const u16* p = pb(c, r);
+        if (p) pB[q] = p + ch * 8;
+      }
+    }
     const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-    auto fetch = [&](int k0) {
-#pragma unroll
-      for (int q = 0; q < A_PER_T; ++q) {
-        const int e = tid + q * NT;
-        ra[q] = zero;
-        if (e < A_CH) {
-          const int ch = e & 3, r = (e >> 2) % BM, c = (e >> 2) / BM;
-          const int k = k0 + ch * 8;
-          const u16* p = pa(c, r);
-          if (p && k < Kp) ra[q] = *reinterpret_cast<const uint4*>(p + k);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < B_PER_T; ++q) {
-        const int e = tid + q * NT;
-        rb[q] = zero;
-        if (e < B_CH) {
-          const int ch = e & 3, r = (e >> 2) % BN, c = (e >> 2) / BN;
-          const int k = k0 + ch * 8;
-          const u16* p = pb(c, r);
-          if (p && k < Kp) rb[q] = *reinterpret_cast<const uint4*>(p + k);
-        }
-      }
-    };
+    // (a branch, not "cond ? *p : zero": the select would load through a pointer to
+    // the local zero, which moves the staging registers to scratch memory)
+#define AFQ_BF16_FETCH(k0)                                                                     \
+  {                                                                                            \
+    _Pragma("unroll") for (int q = 0; q < A_PER_T; ++q) {                                      \
+      const int k_ = (k0) + ((tid + q * NT) % CPR) * 8;                                        \
+      ra[q] = zero;                                                                            \
+      if (pA[q] && k_ < Kp) ra[q] = *reinterpret_cast<const uint4*>(pA[q] + (k0));             \
+    }                                                                                          \
+    _Pragma("unroll") for (int q = 0; q < B_PER_T; ++q) {                                      \
+      const int k_ = (k0) + ((tid + q * NT) % CPR) * 8;                                        \
+      rb[q] = zero;                                                                            \
+      if (pB[q] && k_ < Kp) rb[q] = *reinterpret_cast<const uint4*>(pB[q] + (k0));             \
+    }                                                                                          \
+  }
     const int nst = (Kp + BK - 1) / BK;
-    if (nst > 0) fetch(0);
+    if (nst > 0) AFQ_BF16_FETCH(0)
     for (int s = 0; s < nst; ++s) {
 #pragma unroll
       for (int q = 0; q < A_PER_T; ++q) {
         const int e = tid + q * NT;
         if (e < A_CH) {
-          const int ch = e & 3, r = (e >> 2) % BM, c = (e >> 2) / BM;
+          const int ch = e % CPR, r = (e / CPR) % BM, c = (e / CPR) / BM;
           *reinterpret_cast<uint4*>(As + (c * BM + r) * LDK + ch * 8) = ra[q];
         }
       }
@@ -324,34 +341,38 @@ struct MmaBF16 {
       for (int q = 0; q < B_PER_T; ++q) {
         const int e = tid + q * NT;
         if (e < B_CH) {
-          const int ch = e & 3, r = (e >> 2) % BN, c = (e >> 2) / BN;
+          const int ch = e % CPR, r = (e / CPR) % BN, c = (e / CPR) / BN;
           *reinterpret_cast<uint4*>(Bs + (c * BN + r) * LDK + ch * 8) = rb[q];
         }
       }
       __syncthreads();
-      if (s + 1 < nst) fetch((s + 1) * BK);
-      const int koff = 8 * (lane >> 4);
-      bf16x8 a[NA][TM], b[NB][TN];
+      if (s + 1 < nst) AFQ_BF16_FETCH((s + 1) * BK)
 #pragma unroll
-      for (int c = 0; c < NA; ++c)
+      for (int ks = 0; ks < BK / 32; ++ks) {
+        const int koff = ks * 32 + 8 * (lane >> 4);
+        bf16x8 a[NA][TM], b[NB][TN];
+#pragma unroll
+        for (int c = 0; c < NA; ++c)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+            a[c][i] = *reinterpret_cast<const bf16x8*>(As + (c * BM + wm * WM + i * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            b[c][j] = *reinterpret_cast<const bf16x8*>(Bs + (c * BN + wn * WN + j * 16 + (lane & 15)) * LDK + koff);
 #pragma unroll
         for (int i = 0; i < TM; ++i)
-          a[c][i] = *reinterpret_cast<const bf16x8*>(As + (c * BM + wm * WM + i * 16 + (lane & 15)) * LDK + koff);
 #pragma unroll
-      for (int c = 0; c < NB; ++c)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-          b[c][j] = *reinterpret_cast<const bf16x8*>(Bs + (c * BN + wn * WN + j * 16 + (lane & 15)) * LDK + koff);
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
-          if (NB > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[NB - 1][j], acc[i][j], 0, 0, 0);
-          if (NA > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[NA - 1][i], b[0][j], acc[i][j], 0, 0, 0);
-        }
+          for (int j = 0; j < TN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+            if (NB > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[NB - 1][j], acc[i][j], 0, 0, 0);
+            if (NA > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[NA - 1][i], b[0][j], acc[i][j], 0, 0, 0);
+          }
+      }
       __syncthreads();
     }
+#undef AFQ_BF16_FETCH
   }
 
   __device__ __forceinline__ int row(int i, int r) const { return wm * WM + i * 16 + 4 * (lane >> 4) + r; }

@@ -965,6 +965,56 @@ void launch_qr(const DevState& s, hipStream_t st) {
 }
 
 // ------------------------------------------------------------------ exchange
+// Epilogue shared by the FP64 and BF16 exchange kernels.  F holds one g-block of
+// the tile's f = R_g theta for wt walkers: f_w(i, j) at F[i*LDF + 2n*w + 2j] (re, im).
+// The 256 threads split into wt groups of tpw (a power of two) and each group sums
+// f_ij f_ji over its walker's n^2 pairs; LDF is odd so the transposed reads f_ji
+// of consecutive threads fall in different banks.
+struct XPairs {
+  int tpw, wl, lt;
+  __device__ XPairs(int wt) {
+    int t = 256 / wt, p = 1;
+    while (2 * p <= t) p *= 2;
+    tpw = p;
+    wl = threadIdx.x / tpw;
+    lt = threadIdx.x - wl * tpw;
+  }
+  template <class T>
+  __device__ __forceinline__ void add(const T* F, int LDF, int n, int wt, double& accr, double& acci) const {
+    if (wl >= wt) return;
+    const T* Fw = F + wl * 2 * n;
+    const int step_i = tpw / n, step_j = tpw - step_i * n;
+    int i = lt / n, j = lt - i * n;
+    for (; i < n;) {
+      const double ar = Fw[i * LDF + 2 * j], ai = Fw[i * LDF + 2 * j + 1];
+      const double br = Fw[j * LDF + 2 * i], bi = Fw[j * LDF + 2 * i + 1];
+      accr += ar * br - ai * bi;
+      acci += ar * bi + ai * br;
+      i += step_i;
+      j += step_j;
+      if (j >= n) { j -= n; ++i; }
+    }
+  }
+  // group sums; returns true on the thread that holds its walker's total
+  __device__ __forceinline__ bool finish(double& accr, double& acci, double* red) const {
+    const int gsz = tpw < 64 ? tpw : 64;
+    for (int off = gsz / 2; off > 0; off >>= 1) {
+      accr += __shfl_xor(accr, off, 64);
+      acci += __shfl_xor(acci, off, 64);
+    }
+    if (tpw > 64) {
+      const int wave = threadIdx.x >> 6;
+      if ((threadIdx.x & 63) == 0) { red[2 * wave] = accr; red[2 * wave + 1] = acci; }
+      __syncthreads();
+      if (lt == 0) {
+        accr = acci = 0.0;
+        for (int q = wave; q < wave + tpw / 64; ++q) { accr += red[2 * q]; acci += red[2 * q + 1]; }
+      }
+    }
+    return lt == 0;
+  }
+};
+
 constexpr int XC_BM = 128, XC_BN = 128;
 int exchange_tile_g(int n) { return (n >= 1 && n <= XC_BN / 2) ? XC_BM / n : 0; }
 int exchange_tile_w(int n) { return (n >= 1 && n <= XC_BN / 2) ? XC_BN / (2 * n) : 0; }
@@ -989,61 +1039,40 @@ struct XcB {  // B(k, (w,j,c)) = theta_s[w][k][j][c]
 
 // Xp[s][w][gtile] = sum_{g in tile} sum_ij f_ij f_ji,  f = R_{s,g} theta_s.
 // The output tile is reduced one g-block at a time: its n rows are staged in
-// LDS (<= 64 x XC_BN doubles), thread (wl, i) sums row i of walker wl's block
-// and keeps a running total, so f never leaves the workgroup.
+// LDS and summed by XPairs, so f never leaves the workgroup.
 __global__ __launch_bounds__(256, 2) void k_exchange(const double* R, const double* theta, const int* sel,
                                                      double* Xp, int N, int GE, int nw, int W, int64_t Ktot,
                                                      Slot s0, Slot s1, int ngt, int gridx, int gridy,
                                                      int gridz) {
   using Eng = MmaF64<XC_BM, XC_BN, 2, 2, true, true>;
-  constexpr int FT = (XC_BN / 2) * XC_BN;
+  constexpr int LDF = XC_BN + 1;
+  constexpr int FT = (XC_BN / 2) * LDF;
   constexpr int LDSN = Eng::LDS_DOUBLES > FT ? Eng::LDS_DOUBLES : FT;
   __shared__ double lds[LDSN];
+  __shared__ double red[8];
   const TileId tl = tile_id(gridx, gridy, gridz, 8);
   const Slot sl = tl.z == 0 ? s0 : s1;
   const int n = sl.n, gt = XC_BM / n, wt = XC_BN / (2 * n);
   const int g0 = tl.x * gt, w0 = tl.y * wt;
   Eng e;
   e.run(XcA{R, Ktot, sl.koff, n, gt, g0, GE, N}, XcB{theta, sel, Ktot, sl.koff, n, wt, w0, nw, N}, N, lds);
-  const int tid = threadIdx.x, nr = wt * n;
+  const XPairs xp(wt);
   double accr = 0.0, acci = 0.0;
   for (int gl = 0; gl < gt; ++gl) {
     if (g0 + gl >= GE) break;
     e.store([&](int r, int c, double v) {
       const int rr = r - gl * n;
-      if (rr >= 0 && rr < n) lds[rr * XC_BN + c] = v;
+      if (rr >= 0 && rr < n) lds[rr * LDF + c] = v;
     });
     __syncthreads();
-    if (tid < nr) {
-      const int wl = tid / n, i = tid - wl * n;
-      const double* F = lds + wl * 2 * n;  // f(i,j) at F[i*XC_BN + 2j]
-      double sr = 0.0, si = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double ar = F[i * XC_BN + 2 * j], ai = F[i * XC_BN + 2 * j + 1];
-        const double br = F[j * XC_BN + 2 * i], bi = F[j * XC_BN + 2 * i + 1];
-        sr += ar * br - ai * bi;
-        si += ar * bi + ai * br;
-      }
-      accr += sr;
-      acci += si;
-    }
+    xp.add(lds, LDF, n, wt, accr, acci);
     __syncthreads();
   }
-  if (tid < nr) {
-    lds[2 * tid] = accr;
-    lds[2 * tid + 1] = acci;
-  }
-  __syncthreads();
-  if (tid < wt && w0 + tid < nw) {
-    double sr = 0.0, si = 0.0;
-    for (int i = 0; i < n; ++i) {
-      sr += lds[2 * (tid * n + i)];
-      si += lds[2 * (tid * n + i) + 1];
-    }
-    const int w = sel ? sel[w0 + tid] : w0 + tid;
+  if (xp.finish(accr, acci, red) && xp.wl < wt && w0 + xp.wl < nw) {
+    const int w = sel ? sel[w0 + xp.wl] : w0 + xp.wl;
     double* o = Xp + 2 * (((int64_t)tl.z * W + w) * ngt + tl.x);
-    o[0] = sr;
-    o[1] = si;
+    o[0] = accr;
+    o[1] = acci;
   }
 }
 
@@ -1140,10 +1169,12 @@ __global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u
                                                           const int* sel, double* Xp, int Np, int GE, int nw, int W,
                                                           int G, Slot s0, Slot s1, int ctot, int ngt, int gridx,
                                                           int gridy, int gridz) {
-  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS>;
-  constexpr int FT_BYTES = (XC_BN / 2) * XC_BN * 4;
+  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, (NS == 1 ? 64 : 32)>;
+  constexpr int LDF = XC_BN + 1;
+  constexpr int FT_BYTES = (XC_BN / 2) * LDF * 4;
   constexpr int LDSB = Eng::LDS_BYTES > FT_BYTES ? Eng::LDS_BYTES : FT_BYTES;
   __shared__ __attribute__((aligned(16))) unsigned char lds[LDSB];
+  __shared__ double red[8];
   const TileId tl = tile_id(gridx, gridy, gridz, 8);
   const int zs = gridz > 1 ? tl.z : 0;
   const Slot sl = zs == 0 ? s0 : s1;
@@ -1166,46 +1197,23 @@ __global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u
       },
       Np, lds);
   float* F = reinterpret_cast<float*>(lds);
-  const int tid = threadIdx.x, nr = wt * n;
+  const XPairs xp(wt);
   double accr = 0.0, acci = 0.0;
   for (int gl = 0; gl < gt; ++gl) {
     if (g0 + gl >= GE) break;
     e.store([&](int r, int c, float v) {
       const int rr = r - gl * n;
-      if (rr >= 0 && rr < n) F[rr * XC_BN + c] = v;
+      if (rr >= 0 && rr < n) F[rr * LDF + c] = v;
     });
     __syncthreads();
-    if (tid < nr) {
-      const int wl = tid / n, i = tid - wl * n;
-      const float* Fb = F + wl * 2 * n;
-      double sr = 0.0, si = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double ar = Fb[i * XC_BN + 2 * j], ai = Fb[i * XC_BN + 2 * j + 1];
-        const double br = Fb[j * XC_BN + 2 * i], bi = Fb[j * XC_BN + 2 * i + 1];
-        sr += ar * br - ai * bi;
-        si += ar * bi + ai * br;
-      }
-      accr += sr;
-      acci += si;
-    }
+    xp.add(F, LDF, n, wt, accr, acci);
     __syncthreads();
   }
-  double* red = reinterpret_cast<double*>(lds);
-  if (tid < nr) {
-    red[2 * tid] = accr;
-    red[2 * tid + 1] = acci;
-  }
-  __syncthreads();
-  if (tid < wt && w0 + tid < nw) {
-    double sr = 0.0, si = 0.0;
-    for (int i = 0; i < n; ++i) {
-      sr += red[2 * (tid * n + i)];
-      si += red[2 * (tid * n + i) + 1];
-    }
-    const int w = sel ? sel[w0 + tid] : w0 + tid;
+  if (xp.finish(accr, acci, red) && xp.wl < wt && w0 + xp.wl < nw) {
+    const int w = sel ? sel[w0 + xp.wl] : w0 + xp.wl;
     double* o = Xp + 2 * (((int64_t)zs * W + w) * ngt + tl.x);
-    o[0] = sr;
-    o[1] = si;
+    o[0] = accr;
+    o[1] = acci;
   }
 }
 
