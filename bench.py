@@ -8,8 +8,8 @@ and the local energy of every walker.  Inputs are resident in HBM before the
 timed region.  Paths:
 
   fp64               exact FP64 local energy of every walker;
-  mixed_exchange_cv  force bias and VHS in split BF16 (three passes, FP32
-                     accumulation); control variate: the exchange in one BF16 MFMA
+  mixed_exchange_cv  force bias, VHS and Taylor products in split BF16 (three
+                     passes, FP32 accumulation); control variate: the exchange in one BF16 MFMA
                      pass on every live walker, exact FP64 on a random 1/subset_inv
                      of them (Horvitz-Thompson correction, unbiased); the headline;
   mlp_cv             control variate with the learned MLP surrogate, trained on
@@ -123,7 +123,12 @@ def stage_rooflines(spec, path, n_exact):
         passes = 3 if path["surrogate_prec"] == "bf16x3" else 1
         add("exchange_lp", "k_exchange_bf16 (%d-pass BF16 MFMA, surrogate level)" % passes,
             exchange_flops(spec, W), 1, "bf16", walkers=W, passes=passes)
-    add("taylor", "k_vmul (5 complex V t products, FP64 MFMA)", W * sum(40.0 * N * N * n for n in nsp), 5, "fp64")
+    if path.get("fb_vhs_prec", "fp64") == "fp64":
+        add("taylor", "k_taylor (5 complex V t products, FP64 MFMA)", W * sum(40.0 * N * N * n for n in nsp), 5,
+            "fp64")
+    else:
+        add("taylor", "k_taylor_bf16 (5 complex V t products, 3-pass split BF16 MFMA)",
+            W * sum(40.0 * N * N * n for n in nsp), 5, "bf16", passes=3)
     add("onebody", "k_expmul (E psi, FP64 MFMA)", W * sum(8.0 * N * N * n for n in nsp), 2 * len(nsp), "fp64")
     if path.get("fb_vhs_prec", "fp64") == "fp64":
         add("force_bias", "k_force_bias (FP64 MFMA)", W * sum(4.0 * n * N * G for n in nsp), 1, "fp64")
@@ -193,7 +198,8 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-surrogate path")
     ap.add_argument("--fp64-propagation", action="store_true",
-                    help="mixed path: keep the force bias and VHS in FP64 (default: split BF16)")
+                    help="mixed path: keep the force bias, VHS and Taylor products in FP64 "
+                         "(default: split BF16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
@@ -228,7 +234,8 @@ def main():
         # random 1/subset_inv of the walkers (Horvitz-Thompson correction)
         pp = AFQ_PREC_FP64 if a.fp64_propagation else AFQ_PREC_BF16X3
         xcv = run_path(ham, spec, a.warmup + a.steps, a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
-                       prec_fb=pp, prec_vhs=pp, prec_exchange=AFQ_PREC_FP64, surrogate_kind=AFQ_SURR_EXCHANGE,
+                       prec_fb=pp, prec_vhs=pp, prec_prop=pp, prec_exchange=AFQ_PREC_FP64,
+                       surrogate_kind=AFQ_SURR_EXCHANGE,
                        surrogate_prec=AFQ_PREC_BF16, subset_inv=a.subset_inv, risk_frac=0.0, shadow_exact=True)
         nx = (W + a.subset_inv - 1) // a.subset_inv
         xcv.update(exchange_prec="fp64", surrogate_prec="bf16", n_exact=nx,

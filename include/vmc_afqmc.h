@@ -129,6 +129,10 @@ int afq_destroy(afq_ctx* ctx);
  * AFQ_PREC_BF16 (one pass).  Overlaps, Green's functions, the propagation,
  * weights and QR stay FP64.  ref_compat contexts need prec_vhs = AFQ_PREC_FP64. */
 int afq_set_precision(afq_ctx* ctx, int32_t prec_fb, int32_t prec_vhs, int32_t prec_exchange);
+/* Arithmetic of the five Taylor products V t of the propagation: AFQ_PREC_FP64
+ * (default) or AFQ_PREC_BF16X3 (split-BF16 MFMA, FP32 accumulation; V and t stay
+ * FP64 in memory and the series is accumulated in FP64). */
+int afq_set_prop_precision(afq_ctx* ctx, int32_t prec_prop);
 
 /* Number of Cholesky vectors used by the local energy (default nchol). */
 int afq_set_nchol_ene(afq_ctx* ctx, int32_t nchol_ene);
@@ -296,6 +300,7 @@ typedef struct afq_run_params {
                           * difference of the two block energies on identical walkers */
   int32_t surrogate_kind; /* AFQ_SURR_MLP (trained first) | AFQ_SURR_EXCHANGE (no training) */
   int32_t surrogate_prec; /* precision of the AFQ_SURR_EXCHANGE level                      */
+  int32_t prec_prop;      /* Taylor products (afq_set_prop_precision)                       */
 } afq_run_params;
 
 typedef struct afq_run_result {

@@ -51,11 +51,13 @@ def test_force_bias_bf16x3_vs_fp64(name):
     assert err.max() < 2e-5
 
 
+@pytest.mark.parametrize("prop", [False, True])
 @pytest.mark.parametrize("name", ["uhf_uhf", "c0_rhf", "uhf_n60", "c1_small"])
-def test_mixed_step_vs_fp64(name):
+def test_mixed_step_vs_fp64(name, prop):
     """Three steps from the same walkers with the same draws: force bias, VHS and
-    exchange in BF16x3 against all-FP64.  The propagated walkers, weights and
-    local energies stay within 1e-5 (relative) of the FP64 path."""
+    exchange (and with prop, the Taylor products) in BF16x3 against all-FP64.  The
+    propagated walkers, weights and local energies stay within 1e-5 (relative) of
+    the FP64 path."""
     W = 6
     h, wrhf, ctx, orc = build(name, W)
     h2, _, ctx2, _ = build(name, W)
@@ -65,6 +67,8 @@ def test_mixed_step_vs_fp64(name):
     eshift = (o[0] / o[1]).real
     ctx.set_walkers(dets)
     ctx2.set_precision(prec_fb=AFQ_PREC_BF16X3, prec_vhs=AFQ_PREC_BF16X3, prec_exchange=AFQ_PREC_BF16X3)
+    if prop:
+        ctx2.set_prop_precision(AFQ_PREC_BF16X3)
     ctx2.set_walkers(dets)
     for step in range(3):
         x = rng.standard_normal((W, h.nchol))
@@ -79,5 +83,5 @@ def test_mixed_step_vs_fp64(name):
     rd = np.max(np.abs(d2 - d1)) / np.max(np.abs(d1))
     rw = np.max(np.abs(w2[live] - w1[live]) / w1[live])
     re = np.max(np.abs(e2[live] - e1[live]) / np.abs(e1[live]))
-    print(name, "mixed vs fp64 after 3 steps: dets %.2e weights %.2e E_L %.2e" % (rd, rw, re))
+    print(name, prop, "mixed vs fp64 after 3 steps: dets %.2e weights %.2e E_L %.2e" % (rd, rw, re))
     assert rd < 1e-5 and rw < 1e-5 and re < 1e-5

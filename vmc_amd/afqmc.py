@@ -146,6 +146,9 @@ class Context:
     def set_precision(self, prec_fb=AFQ_PREC_FP64, prec_vhs=AFQ_PREC_FP64, prec_exchange=AFQ_PREC_FP64):
         self._chk(lib().afq_set_precision(self.ptr, prec_fb, prec_vhs, prec_exchange), "afq_set_precision")
 
+    def set_prop_precision(self, prec_prop=AFQ_PREC_FP64):
+        self._chk(lib().afq_set_prop_precision(self.ptr, prec_prop), "afq_set_prop_precision")
+
     def set_nchol_ene(self, n):
         self._chk(lib().afq_set_nchol_ene(self.ptr, int(n)), "afq_set_nchol_ene")
 
@@ -332,7 +335,7 @@ class RunParams(C.Structure):
         ("surrogate_train_steps", C.c_int32), ("surrogate_hidden", C.c_int32),
         ("surrogate_epochs", C.c_int32), ("subset_inv", C.c_int32), ("fallback_rms", C.c_double),
         ("risk_frac", C.c_double), ("shadow_exact", C.c_int32), ("surrogate_kind", C.c_int32),
-        ("surrogate_prec", C.c_int32),
+        ("surrogate_prec", C.c_int32), ("prec_prop", C.c_int32),
     ]
 
 
@@ -354,7 +357,7 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   prec_vhs=AFQ_PREC_FP64, prec_exchange=AFQ_PREC_FP64,
                   energy_mode=AFQ_ENERGY_EXACT, device=0, surrogate_train_steps=0, surrogate_hidden=256,
                   surrogate_epochs=20, subset_inv=16, fallback_rms=0.0, risk_frac=0.05, shadow_exact=False,
-                  surrogate_kind=AFQ_SURR_MLP, surrogate_prec=AFQ_PREC_BF16):
+                  surrogate_kind=AFQ_SURR_MLP, surrogate_prec=AFQ_PREC_BF16, prec_prop=AFQ_PREC_FP64):
     """calcMixedEstimatorLongProp on the GPU; returns (weights, energies, RunResult)."""
     L = lib()
     if not hasattr(L, "_run_bound"):
@@ -366,7 +369,7 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   int(ref_compat), cholesky_threshold, ene0_guess, weight_cap, max_steps, rng_mode,
                   warmup_steps, prec_fb, prec_vhs, prec_exchange, energy_mode, device,
                   surrogate_train_steps, surrogate_hidden, surrogate_epochs, subset_inv, fallback_rms,
-                  risk_frac, int(shadow_exact), surrogate_kind, surrogate_prec)
+                  risk_frac, int(shadow_exact), surrogate_kind, surrogate_prec, prec_prop)
     w = np.zeros(nsweeps)
     e = np.zeros(nsweeps)
     res = RunResult()

@@ -587,6 +587,16 @@ int afq_set_precision(afq_ctx* c, int32_t prec_fb, int32_t prec_vhs, int32_t pre
   AFQ_GUARD_END(c)
 }
 
+int afq_set_prop_precision(afq_ctx* c, int32_t prec_prop) {
+  if (!c) return AFQ_ERR_ARG;
+  if (prec_prop != AFQ_PREC_FP64 && prec_prop != AFQ_PREC_BF16X3)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "propagation precision must be FP64 or BF16X3");
+  if (prec_prop != AFQ_PREC_FP64 && c->s.ref_compat)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "ref_compat needs the FP64 propagation");
+  c->s.prec_prop = prec_prop;
+  return AFQ_OK;
+}
+
 int afq_set_nchol_ene(afq_ctx* c, int32_t n) {
   if (!c) return AFQ_ERR_ARG;
   if (n < 0 || n > c->s.G) return fail(c, AFQ_ERR_ARG, "nchol_ene out of range");

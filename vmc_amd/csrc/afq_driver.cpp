@@ -57,6 +57,7 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   rc = afq_create(ham, left, &cp, &g.c);
   if (rc) return rc;
   afq_ctx* c = g.c;
+  if (p->prec_prop != AFQ_PREC_FP64 && (rc = afq_set_prop_precision(c, p->prec_prop))) return rc;
   const size_t ms = (size_t)N * (na + nb);
   std::vector<double> ref(2 * ms, 0.0), init(2 * ms, 0.0);
   for (int i = 0; i < N; ++i) {

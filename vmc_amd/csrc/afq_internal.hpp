@@ -82,6 +82,7 @@ struct DevState {
   unsigned short* Rb_hi = nullptr; unsigned short* Rb_lo = nullptr;  // [slot][g][i][Np]
   unsigned short* Tb_hi = nullptr; unsigned short* Tb_lo = nullptr;  // [w][re/im][slot][j][Np]
   int prec_fb = AFQ_PREC_FP64, prec_vhs = AFQ_PREC_FP64;
+  int prec_prop = AFQ_PREC_FP64;   // Taylor products V t
   int Gp = 0;               // G padded to a multiple of 8
   unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
   unsigned short* Fb_hi = nullptr; unsigned short* Fb_lo = nullptr;  // fields, [w][re/im][Gp]

@@ -394,6 +394,126 @@ __global__ __launch_bounds__(512) void k_taylor(const double* P, const double* t
       }
 }
 
+// The same product with split-BF16 operands (x = hi + lo, three MFMA passes
+// hi*hi + hi*lo + lo*hi, FP32 accumulation): P and t are read in FP64 and split
+// while staging into LDS.  A K stage is 16 orbitals = 32 real K rows, one
+// v_mfma_f32_16x16x32_bf16 deep; LDS keeps A as [row][k] and B as [col][k].
+constexpr int TB_KC = 16, TB_K = 2 * TB_KC, TB_LDK = TB_K + 8;
+__global__ __launch_bounds__(512) void k_taylor_bf16(const double* P, const double* tsrc, double* tdst, double* acc,
+                                                     const double* weights, int N, int na, int nb, int64_t offb,
+                                                     int64_t wstride, double scale, int gridx, int gridy, int W) {
+  __shared__ __attribute__((aligned(16))) u16 As[2][TY_BM * TB_LDK];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][TY_BN * TB_LDK];
+  const TileId tl = tile_id(gridx, gridy, W, gridy);
+  const int w = tl.z;
+  if (weights[w] <= 1e-8) return;
+  const int row0 = tl.y * TY_BM, q0 = tl.x * TY_BNC, ncolc = na + nb;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 2, wn = wave & 3;
+  const double* Pre = P + (int64_t)(2 * w) * N * N;
+  const double* Pim = Pre + (int64_t)N * N;
+  const double* src = tsrc + (int64_t)w * wstride;
+  // A: 128 rows x 16 orbitals of P_re and P_im: thread -> (row, pair of orbitals), 2 per thread
+  // B: 16 orbitals x 128 complex columns of t: thread -> (orbital, column), 4 per thread
+  double2 pre[2], pim[2], tb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = tid + 512 * h, r = e >> 3, kp = (e & 7) * 2, i = row0 + r, k = k0 + kp;
+      pre[h] = double2{0.0, 0.0};
+      pim[h] = double2{0.0, 0.0};
+      if (i < N) {
+        const int64_t o = (int64_t)i * N + k;
+        if (k < N) { pre[h].x = Pre[o]; pim[h].x = Pim[o]; }
+        if (k + 1 < N) { pre[h].y = Pre[o + 1]; pim[h].y = Pim[o + 1]; }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int e = tid + 512 * h, kk = e & (TB_KC - 1), qc = e >> 4;
+      const int kr = k0 + kk, q = q0 + qc;
+      tb[h] = double2{0.0, 0.0};
+      if (kr < N && q < ncolc) {
+        const double* rowp = q < na ? src + 2 * ((int64_t)kr * na + q) : src + 2 * (offb + (int64_t)kr * nb + (q - na));
+        tb[h] = *reinterpret_cast<const double2*>(rowp);
+      }
+    }
+  };
+  auto put = [&](u16* hi, u16* lo, int idx, double x) {
+    u16 a, b;
+    split_bf16(x, a, b);
+    hi[idx] = a;
+    lo[idx] = b;
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = tid + 512 * h, r = e >> 3, kp = (e & 7) * 2;
+      // [A_re | A_im] = [-P_im | P_re]
+      put(As[0], As[1], r * TB_LDK + kp, -pim[h].x);
+      put(As[0], As[1], r * TB_LDK + kp + 1, -pim[h].y);
+      put(As[0], As[1], r * TB_LDK + TB_KC + kp, pre[h].x);
+      put(As[0], As[1], r * TB_LDK + TB_KC + kp + 1, pre[h].y);
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int e = tid + 512 * h, kk = e & (TB_KC - 1), qc = e >> 4;
+      // columns (q, re) and (q, im): [[T_re, T_im], [-T_im, T_re]]
+      put(Bs[0], Bs[1], (2 * qc) * TB_LDK + kk, tb[h].x);
+      put(Bs[0], Bs[1], (2 * qc + 1) * TB_LDK + kk, tb[h].y);
+      put(Bs[0], Bs[1], (2 * qc) * TB_LDK + TB_KC + kk, -tb[h].y);
+      put(Bs[0], Bs[1], (2 * qc + 1) * TB_LDK + TB_KC + kk, tb[h].x);
+    }
+  };
+  f32x4 c[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nst = (N + TB_KC - 1) / TB_KC;
+  fetch(0);
+  for (int st = 0; st < nst; ++st) {
+    stage();
+    __syncthreads();
+    if (st + 1 < nst) fetch((st + 1) * TB_KC);
+    const int koff = 8 * (lane >> 4);
+    bf16x8 a[2][4], b[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[h][i] = *reinterpret_cast<const bf16x8*>(As[h] + (wm * 64 + i * 16 + (lane & 15)) * TB_LDK + koff);
+        b[h][i] = *reinterpret_cast<const bf16x8*>(Bs[h] + (wn * 64 + i * 16 + (lane & 15)) * TB_LDK + koff);
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], c[i][j], 0, 0, 0);
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], c[i][j], 0, 0, 0);
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], c[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  double* dst = tdst + (int64_t)w * wstride;
+  double* ac = acc + (int64_t)w * wstride;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;  // bf16 D layout
+        const int cr = wn * 64 + j * 16 + (lane & 15);
+        const int q = q0 + (cr >> 1), comp = cr & 1;
+        if (row < N && q < ncolc) {
+          const int64_t a = (q < na ? 2 * ((int64_t)row * na + q) : 2 * (offb + (int64_t)row * nb + (q - na))) + comp;
+          const double x = (double)c[i][j][r] * scale;
+          dst[a] = x;
+          ac[a] += x;
+        }
+      }
+}
+
 // psi <- E (sum_{k<=5} V^k / k!) E psi  for every live walker
 void launch_expmul_pre(const DevState& s, hipStream_t st) {
   const int64_t ws = 2 * s.ms;
@@ -415,7 +535,11 @@ void launch_taylor(const DevState& s, hipStream_t st) {
   const bool wide = n2a + n2b > 128;
   const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
   for (int p = 1; p <= 5; ++p) {
-    if (wide) {
+    if (s.prec_prop != AFQ_PREC_FP64) {
+      const int gx = (int)cdivu(na + nb, TY_BNC), gy = (int)cdivu(s.N, TY_BM);
+      hipLaunchKernelGGL(k_taylor_bf16, dim3(gx * gy * s.W), dim3(512), 0, st, s.P, src, dst, s.accb, s.weights,
+                         s.N, na, nb, s.det_off[1], ws, sq / p, gx, gy, s.W);
+    } else if (wide) {
       const int gx = (int)cdivu(na + nb, TY_BNC), gy = (int)cdivu(s.N, TY_BM);
       if (s.N % 2 == 0)
         hipLaunchKernelGGL(k_taylor<true>, dim3(gx * gy * s.W), dim3(512), 0, st, s.P, src, dst, s.accb, s.weights,
