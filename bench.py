@@ -96,50 +96,99 @@ def run_path(ham, spec, total_steps, warmup, **kw):
                 shadow_wall_s=float(res.shadow_wall_s), live_frac=float(res.live_frac))
 
 
+HBM_PEAK_GBS = 8000.0
+
+
 def stage_rooflines(spec, path, n_exact):
-    """Algorithmic FLOP/s of each stage of one step (per launch for the kernels that
-    launch several times) against the MFMA peak of the precision it runs in.
+    """Roofline of each stage of one step (per launch for the kernels that launch
+    several times).  Algorithmic FLOPs and HBM bytes per launch give two lower
+    bounds on the time, FLOPs x passes / MFMA peak and bytes / 8 TB/s; the larger
+    one names the bound, and `achieved` is the algorithmic rate in that unit.
     n_exact: walkers whose exchange runs in FP64 in one step."""
     N, G, W = spec["norb"], spec["nchol"], spec["nwalk"]
     T = N * (N + 1) / 2
     nsp = [spec["nalpha"]] if spec["walker_rhf"] else [spec["nalpha"], spec["nbeta"]]
+    ncol = sum(nsp)
+    ms = N * ncol                      # complex elements of a walker's orbitals
+    live = path.get("live_frac", 1.0)
+    Np = (N + 7) // 8 * 8
     st = path["stage_ms_per_step"]
     out = {}
-    def add(name, kernel, flops, launches, prec, **extra):
-        ms = st.get(name, 0.0) / launches
-        if ms <= 0:
+
+    def add(name, kernel, flops, nbytes, launches, prec, symbol=None, passes=1, **extra):
+        ms_l = st.get(name, 0.0) / launches
+        if ms_l <= 0:
             return
-        ach = flops / launches / (ms * 1e-3) / 1e12
-        out[name] = dict(kernel=kernel, launch_ms=ms, flops_per_launch=flops / launches,
-                         achieved=ach, peak=PEAKS[prec], frac=ach / PEAKS[prec], unit="TFLOP/s",
-                         bound="mfma", precision=prec, **extra)
+        f, b = flops / launches, nbytes / launches
+        t_mfma = f * passes / (PEAKS[prec] * 1e12)
+        t_hbm = b / (HBM_PEAK_GBS * 1e9)
+        if t_hbm > t_mfma:
+            ach = b / (ms_l * 1e-3) / 1e9
+            r = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS)
+        else:
+            ach = f / (ms_l * 1e-3) / 1e12
+            r = dict(bound="mfma", achieved=ach, peak=PEAKS[prec], unit="TFLOP/s", frac=ach / PEAKS[prec])
+        r.update(kernel=kernel, symbol=symbol, launch_ms=ms_l, flops_per_launch=f, bytes_per_launch=b,
+                 precision=prec, passes=passes, **extra)
+        out[name] = r
+
+    # exchange: the rotated Cholesky once per launch + theta of each walker measured
     if path["exchange_prec"] == "fp64":
-        add("exchange", "k_exchange (FP64 MFMA)", exchange_flops(spec, n_exact), 1, "fp64",
-            walkers=n_exact)
+        add("exchange", "k_exchange (FP64 MFMA)", exchange_flops(spec, n_exact),
+            8.0 * G * ms + 16.0 * ms * n_exact, 1, "fp64", symbol="k_exchange", walkers=n_exact)
     else:
-        add("exchange", "k_exchange_bf16 (3-pass split BF16 MFMA)", exchange_flops(spec, n_exact), 1, "bf16",
-            walkers=n_exact, passes=3)
+        add("exchange", "k_exchange_bf16 (3-pass split BF16 MFMA)", exchange_flops(spec, n_exact),
+            4.0 * G * ncol * Np + 8.0 * ncol * Np * n_exact, 1, "bf16", symbol="void k_exchange_bf16<2>",
+            passes=3, walkers=n_exact)
     if path.get("surrogate_prec"):
         passes = 3 if path["surrogate_prec"] == "bf16x3" else 1
+        nl = W * live
         add("exchange_lp", "k_exchange_bf16 (%d-pass BF16 MFMA, surrogate level)" % passes,
-            exchange_flops(spec, W), 1, "bf16", walkers=W, passes=passes)
+            exchange_flops(spec, nl), 2.0 * (2 if passes == 3 else 1) * (G * ncol * Np + 2 * ncol * Np * nl),
+            1, "bf16", symbol="void k_exchange_bf16<%d>" % (2 if passes == 3 else 1), passes=passes, walkers=nl)
+    # Taylor: V (both planes) once, t in, t out, the accumulator read and written, per walker and product
+    tay_bytes = W * (16.0 * N * N + 4 * 16.0 * ms) * 5
     if path.get("fb_vhs_prec", "fp64") == "fp64":
-        add("taylor", "k_taylor (5 complex V t products, FP64 MFMA)", W * sum(40.0 * N * N * n for n in nsp), 5,
-            "fp64")
+        wide = ncol > 64
+        add("taylor", "k_taylor (5 complex V t products, FP64 MFMA)", W * 40.0 * N * N * ncol, tay_bytes, 5,
+            "fp64", symbol=("void k_taylor<%s>" % ("true" if N % 2 == 0 else "false")) if wide
+            else "void k_vmul<64, 128, 2, 2>")
+        add("force_bias", "k_force_bias (FP64 MFMA)", W * 4.0 * ncol * N * G, 8.0 * G * ms + W * 16.0 * (ms + G),
+            1, "fp64", symbol="k_force_bias")
+        add("vhs", "k_vhs (FP64 MFMA)", W * 4.0 * T * G, 8.0 * G * T + W * 16.0 * (G + N * N), 1, "fp64",
+            symbol="k_vhs")
     else:
-        add("taylor", "k_taylor_bf16 (5 complex V t products, 3-pass split BF16 MFMA)",
-            W * sum(40.0 * N * N * n for n in nsp), 5, "bf16", passes=3)
-    add("onebody", "k_expmul (E psi, FP64 MFMA)", W * sum(8.0 * N * N * n for n in nsp), 2 * len(nsp), "fp64")
-    if path.get("fb_vhs_prec", "fp64") == "fp64":
-        add("force_bias", "k_force_bias (FP64 MFMA)", W * sum(4.0 * n * N * G for n in nsp), 1, "fp64")
-        add("vhs", "k_vhs (FP64 MFMA)", W * 4.0 * T * G, 1, "fp64")
-    else:
-        add("force_bias", "k_theta_split + k_force_bias_bf16<2> (3-pass split BF16 MFMA)",
-            W * sum(4.0 * n * N * G for n in nsp), 1, "bf16", passes=3)
-        add("vhs", "k_fields_split + k_vhs_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * T * G, 1, "bf16", passes=3)
-    add("green", "k_green (FP64 MFMA + Gauss-Jordan)", W * sum(4 * n * n * N + 16 * n ** 3 + 8 * N * n * n
-                                                              for n in nsp), 1, "fp64")
+        add("taylor", "k_taylor_bf16 (5 complex V t products, 3-pass split BF16 MFMA)", W * 40.0 * N * N * ncol,
+            tay_bytes, 5, "bf16", symbol="k_taylor_bf16", passes=3)
+        add("force_bias", "k_theta_split + k_force_bias_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * ncol * N * G,
+            4.0 * G * ncol * Np + W * (16.0 * ms + 8.0 * ncol * Np + 16.0 * G), 1, "bf16",
+            symbol="void k_force_bias_bf16<2>", passes=3)
+        add("vhs", "k_fields_split + k_vhs_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * T * G,
+            4.0 * G * T + W * (16.0 * G + 16.0 * N * N), 1, "bf16", symbol="void k_vhs_bf16<2>", passes=3)
+    add("onebody", "k_expmul (E psi, FP64 MFMA)", W * 8.0 * N * N * ncol, W * 3 * 16.0 * ms, 2 * len(nsp), "fp64",
+        symbol="k_expmul")
+    add("green", "k_green (FP64 MFMA + Gauss-Jordan)",
+        W * sum(4 * n * n * N + 16 * n ** 3 + 8 * N * n * n for n in nsp), W * 2 * 16.0 * ms, 1, "fp64",
+        symbol="k_green")
     return out
+
+
+PROFILE_HBM = os.path.join(ROOT, "profiles", "walker_step_N300_G1200_W4096_hbm_counters.md")
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) of a kernel from the
+    committed rocprofv3 counter summary of this workload, or None."""
+    if workload != "c4" or not os.path.exists(PROFILE_HBM):
+        return None
+    for line in open(PROFILE_HBM):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == kernel:
+            try:
+                return (float(cells[3]) + float(cells[5])) * 2 ** 20
+            except ValueError:
+                return None
+    return None
 
 
 def cv_summary(mx, fp64):
@@ -181,6 +230,34 @@ def cpu_baseline(ham, spec, threads):
                        % (threads, nsteps, spec["ortho"], secs, threads))
 
 
+def micro(kind, workload, nwalk=1024, calls=8):
+    """Kernel microbenchmark for profiling: one context of `nwalk` walkers near the
+    trial, `calls` measurements with the exchange in the given precision."""
+    from vmc_amd.abi import AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64
+    from vmc_amd.afqmc import Context
+    from vmc_amd.synthetic import workload as get_workload
+    prec = {"exchange-bf16": AFQ_PREC_BF16, "exchange-bf16x3": AFQ_PREC_BF16X3, "exchange-fp64": AFQ_PREC_FP64}[kind]
+    h, spec = get_workload(workload)
+    ctx = Context(h, nwalk, spec["walker_rhf"])
+    ctx.prep_prop(h.phi_up, h.phi_dn, spec["dt"])
+    rng = np.random.default_rng(0)
+    dets = []
+    for _ in range(nwalk):
+        up = h.phi_up + 0.02 * (rng.standard_normal(h.phi_up.shape) + 1j * rng.standard_normal(h.phi_up.shape))
+        dn = h.phi_dn + 0.02 * (rng.standard_normal(h.phi_dn.shape) + 1j * rng.standard_normal(h.phi_dn.shape))
+        dets.append(h.serial_det(up, dn))
+    ctx.set_walkers(np.array(dets))
+    ctx.set_precision(prec_exchange=prec)
+    ctx.measure()
+    ctx.sync()
+    t0 = time.time()
+    for _ in range(calls):
+        ctx.measure()
+    ctx.sync()
+    ms = (time.time() - t0) / calls * 1e3
+    print(json.dumps({"micro": kind, "workload": workload, "nwalk": nwalk, "ms_per_measure": ms}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +281,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 1 GiB plain-copy kernel (HBM counter calibration)")
+    ap.add_argument("--micro", choices=["exchange-bf16", "exchange-bf16x3", "exchange-fp64"],
+                    help="kernel microbenchmark for profiling (1024 walkers, 8 measurements) instead of the bench")
     a = ap.parse_args()
+    if a.micro:
+        micro(a.micro, a.workload)
+        return
     if a.gpus != 1:
         print("bench.py: this build runs one GPU holding one rank's walker population; "
               "--gpus %d is not supported" % a.gpus,
@@ -224,7 +306,8 @@ def main():
     log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
     paths = {}
     fp64 = run_path(ham, spec, a.warmup + a.steps, a.warmup)
-    fp64.update(exchange_prec="fp64", n_exact=W)
+    # the exchange runs on the live walkers only (dead walkers get no local energy)
+    fp64.update(exchange_prec="fp64", n_exact=W * fp64["live_frac"])
     paths["fp64"] = fp64
     log("[bench] fp64: %.1f walker-steps/s, %.1f ms/step, live %.3f"
         % (fp64["walker_steps_per_s"], fp64["ms_per_step"], fp64["live_frac"]))
@@ -237,7 +320,7 @@ def main():
                        prec_fb=pp, prec_vhs=pp, prec_prop=pp, prec_exchange=AFQ_PREC_FP64,
                        surrogate_kind=AFQ_SURR_EXCHANGE,
                        surrogate_prec=AFQ_PREC_BF16, subset_inv=a.subset_inv, risk_frac=0.0, shadow_exact=True)
-        nx = (W + a.subset_inv - 1) // a.subset_inv
+        nx = W * xcv["live_frac"] / a.subset_inv
         xcv.update(exchange_prec="fp64", surrogate_prec="bf16", n_exact=nx,
                    fb_vhs_prec="fp64" if a.fp64_propagation else "bf16x3")
         xcv.update(cv_summary(xcv, fp64))
@@ -253,8 +336,8 @@ def main():
                            prec_exchange=AFQ_PREC_FP64, surrogate_kind=AFQ_SURR_MLP, surrogate_train_steps=T,
                            surrogate_hidden=256, surrogate_epochs=a.epochs, subset_inv=a.subset_inv,
                            risk_frac=a.risk_frac, shadow_exact=True)
-            nrisk = int(np.ceil(a.risk_frac * W))
-            mlp.update(exchange_prec="fp64", n_exact=nrisk + (W - nrisk + a.subset_inv - 1) // a.subset_inv)
+            nl = W * mlp["live_frac"]
+            mlp.update(exchange_prec="fp64", n_exact=a.risk_frac * nl + (1 - a.risk_frac) * nl / a.subset_inv)
             mlp.update(cv_summary(mlp, fp64))
             paths["mlp_cv"] = mlp
             log("[bench] mlp surrogate (train rmse %.3g, valid %.3g Ha): %.1f walker-steps/s, %.1f ms/step, "
@@ -270,7 +353,9 @@ def main():
     dom_name = max(shares, key=shares.get)
     dom = dict(best["rooflines"][dom_name])
     dom["stage"] = dom_name
-    dom["traffic"] = None
+    dom["traffic"] = profiled_traffic(a.workload, dom.get("symbol"))
+    if dom["traffic"] is not None:
+        dom["traffic_source"] = os.path.relpath(PROFILE_HBM, ROOT)
     cpu = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
