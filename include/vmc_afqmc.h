@@ -255,6 +255,38 @@ int afq_set_energy_clip(afq_ctx* ctx, const double* center, double halfwidth);
  *  fallbacks} of the last surrogate measurement */
 int afq_cv_stats(afq_ctx* ctx, double* out6);
 
+/* ---------------------------------------------------------------- tuning
+ * Tile-shape autotuner (no counterpart in the reference): times the alternative
+ * Taylor-product kernels (64 x 128 real tiles; 128 x 128 complex tiles) on the
+ * context's current walkers and keeps the faster.  choice (nullable) receives 1 or
+ * 2 (0: nothing to choose in the current precision); ms2 (nullable) the two times. */
+int afq_autotune(afq_ctx* ctx, int32_t* choice, double* ms2);
+/* 0: by shape (default), 1: 64 x 128 real tiles, 2: 128 x 128 complex tiles */
+int afq_set_taylor_variant(afq_ctx* ctx, int32_t variant);
+
+/* ---------------------------------------------------------------- batched systems
+ * Many independent populations of different sizes on one GPU (no counterpart in the
+ * reference, which runs one system per process).  afq_batch_create makes one
+ * context per system; prepare each through afq_batch_context (afq_prep_prop,
+ * afq_set_walkers, precision, afq_autotune), then afq_batch_capture records one step
+ * of every system -- propagation, QR when step % ortho_steps == 0, exact energies of
+ * every walker, block totals with the clip, comb -- into a hipGraph with one branch
+ * per system.  afq_batch_step replays it: per system the energy shift, comb uniform
+ * and clip centre go in, {sum w, sum w clip(E_L)} comes out (one copy each way).
+ * The device RNG draws Philox(seed + 7919 i, step, walker) for system i, the step
+ * counter starting at first_step.  After the capture the systems' buffers must not
+ * be re-pointed: drive them only through afq_batch_step (or capture again). */
+typedef struct afq_batch afq_batch;
+int afq_batch_create(int32_t nsys, const afq_ham* hams, const afq_trial* trials, const afq_params* params,
+                     afq_batch** out);
+int afq_batch_destroy(afq_batch* b);
+int afq_batch_context(afq_batch* b, int32_t i, afq_ctx** ctx);
+int afq_batch_capture(afq_batch* b, uint64_t seed, int32_t ortho_steps, double clip_halfwidth,
+                      uint64_t first_step);
+/* graph = 0 issues the same kernels without the graph (for comparison) */
+int afq_batch_step(afq_batch* b, int64_t step, const double* eshift, const double* zeta, const double* center,
+                   int32_t graph, double* totals);
+
 /* ---------------------------------------------------------------- driver
  * calcMixedEstimatorLongProp (DQMC/MixedEstimator.cpp:309-699) on one GPU.
  * nranks virtual ranks of nwalk walkers each reproduce an `mpirun -np nranks`

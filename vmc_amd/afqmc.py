@@ -128,11 +128,24 @@ class Context:
             raise AfqError("afq_create(%d): %s" % (rc, lib().afq_last_error(None).decode()))
         self.ptr = p
 
+    @classmethod
+    def borrowed(cls, ham, nwalk, walker_rhf, ptr):
+        """A view of a context owned elsewhere (a batch's system); never destroyed here."""
+        self = cls.__new__(cls)
+        self.h = ham
+        self.nwalk = int(nwalk)
+        self.ptr = ptr
+        self._borrowed = True
+        return self
+
     def _chk(self, rc, what):
         if rc != 0:
             raise AfqError("%s(%d): %s" % (what, rc, lib().afq_last_error(self.ptr).decode()))
 
     def close(self):
+        if getattr(self, "_borrowed", False):
+            self.ptr = None
+            return
         if getattr(self, "ptr", None):
             lib().afq_destroy(self.ptr)
             self.ptr = None
@@ -317,6 +330,75 @@ class Context:
         o = np.zeros(6)
         self._chk(lib().afq_cv_stats(self.ptr, dptr(o)), "afq_cv_stats")
         return dict(nsel=o[0], rms=o[1], fallback=o[2], var_sub=o[3], mean_resid=o[4], n_fallback=o[5])
+
+
+class Batch:
+    """Many systems advanced together (afq_batch_*): one context per system, one
+    hipGraph launch per step."""
+
+    def __init__(self, hams, walker_rhf, nwalk, device=0):
+        L = lib()
+        if not hasattr(L, "_batch_bound"):
+            vp = C.c_void_p
+            L.afq_batch_create.argtypes = [C.c_int32, C.POINTER(AfqHam), C.POINTER(AfqTrial), C.POINTER(AfqParams),
+                                           C.POINTER(vp)]
+            L.afq_batch_destroy.argtypes = [vp]
+            L.afq_batch_context.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+            L.afq_batch_capture.argtypes = [vp, C.c_uint64, C.c_int32, C.c_double, C.c_uint64]
+            L.afq_batch_step.argtypes = [vp, C.c_int64, DP, DP, DP, C.c_int32, DP]
+            L.afq_autotune.argtypes = [vp, C.POINTER(C.c_int32), DP]
+            L.afq_set_taylor_variant.argtypes = [vp, C.c_int32]
+            L._batch_bound = True
+        n = len(hams)
+        self.hams = hams
+        self.n = n
+        hs = (AfqHam * n)(*[h.ham_struct() for h in hams])
+        ts = (AfqTrial * n)(*[h.trial_struct() for h in hams])
+        ps = (AfqParams * n)(*[AfqParams(nwalk, int(walker_rhf), 0, AFQ_PREC_FP64, AFQ_PREC_FP64, AFQ_PREC_FP64,
+                                         device, 0.0) for _ in hams])
+        self.ptr = C.c_void_p()
+        rc = L.afq_batch_create(n, hs, ts, ps, C.byref(self.ptr))
+        if rc:
+            raise AfqError("afq_batch_create(%d): %s" % (rc, L.afq_last_error(None).decode()))
+        self.ctx = []
+        for i, h in enumerate(hams):
+            cp = C.c_void_p()
+            L.afq_batch_context(self.ptr, i, C.byref(cp))
+            self.ctx.append(Context.borrowed(h, nwalk, walker_rhf, cp))
+
+    def autotune(self):
+        out = []
+        for c in self.ctx:
+            ch = C.c_int32(0)
+            ms = np.zeros(2)
+            c._chk(lib().afq_autotune(c.ptr, C.byref(ch), dptr(ms)), "afq_autotune")
+            out.append((ch.value, ms))
+        return out
+
+    def capture(self, seed, ortho_steps, clip_halfwidth, first_step=1):
+        rc = lib().afq_batch_capture(self.ptr, seed, ortho_steps, clip_halfwidth, first_step)
+        if rc:
+            raise AfqError("afq_batch_capture(%d): %s" % (rc, lib().afq_last_error(None).decode()))
+
+    def step(self, step, eshift, zeta, center, graph=True):
+        tot = np.zeros(2 * self.n)
+        args = [np.ascontiguousarray(x, dtype=np.float64) for x in (eshift, zeta, center)]
+        rc = lib().afq_batch_step(self.ptr, step, dptr(args[0]), dptr(args[1]), dptr(args[2]), int(graph),
+                                  dptr(tot))
+        if rc:
+            raise AfqError("afq_batch_step(%d): %s" % (rc, lib().afq_last_error(None).decode()))
+        return tot.reshape(self.n, 2)
+
+    def close(self):
+        if self.ptr:
+            lib().afq_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 AFQ_RNG_REPLAY = 0

@@ -1442,3 +1442,265 @@ int afq_sync(afq_ctx* c) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- autotuner
+// Times the alternative tile shapes of the Taylor products (k_vmul's 64 x 128 real
+// tiles against k_taylor's 128 x 128-complex tiles) on this context's current
+// walkers and keeps the faster one.  The products only write Taylor scratch, so the
+// walker state is unchanged.
+int afq_autotune(afq_ctx* c, int32_t* choice, double* ms_out) {
+  if (!c) return AFQ_ERR_ARG;
+  if (!c->prepped || !c->have_walkers) return fail(c, AFQ_ERR_STATE, "afq_prep_prop and afq_set_walkers first");
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  if (s.prec_prop != AFQ_PREC_FP64) {  // one split-BF16 kernel: nothing to choose
+    if (choice) *choice = 0;
+    return AFQ_OK;
+  }
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "event");
+  hip_check(hipEventCreate(&b), "event");
+  double best = 1e300;
+  int pick = 1;
+  const int keep = s.taylor_variant;
+  try {
+    for (int v = 1; v <= 2; ++v) {
+      s.taylor_variant = v;
+      afq::launch_taylor(s, c->stream);  // warm
+      hip_check(hipEventRecord(a, c->stream), "record");
+      for (int r = 0; r < 3; ++r) afq::launch_taylor(s, c->stream);
+      hip_check(hipEventRecord(b, c->stream), "record");
+      hip_check(hipEventSynchronize(b), "sync");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, a, b), "elapsed");
+      if (ms_out) ms_out[v - 1] = ms / 3.0;
+      if (ms < best) { best = ms; pick = v; }
+    }
+  } catch (...) {
+    s.taylor_variant = keep;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  s.taylor_variant = pick;
+  if (choice) *choice = pick;
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_set_taylor_variant(afq_ctx* c, int32_t v) {
+  if (!c || v < 0 || v > 2) return c ? fail(c, AFQ_ERR_ARG, "taylor variant must be 0, 1 or 2") : AFQ_ERR_ARG;
+  c->s.taylor_variant = v;
+  return AFQ_OK;
+}
+
+// ---------------------------------------------------------------- batched systems
+// Many independent populations (systems of different sizes) advanced together.
+// One step of every system (propagation, optional QR, exact measurement of every
+// walker, block totals with the clip, comb) is captured once into a hipGraph whose
+// branches are the systems' streams; a step is then one graph launch.  The step
+// counter, energy shifts, comb uniforms and clip centres live in device arrays
+// that the host refreshes between launches; block totals come back in one copy.
+struct afq_batch {
+  std::vector<afq_ctx*> ctx;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // [0] plain step, [1] step with QR
+  unsigned long long* d_step = nullptr;
+  double* d_par = nullptr;   // [eshift | zeta | centre] x nsys
+  double* d_tot = nullptr;   // 2 x nsys
+  double* h_par = nullptr;   // pinned staging of d_par
+  double* h_tot = nullptr;   // pinned staging of d_tot
+  double halfw = 0.0;
+  uint64_t seed = 0;
+  int ortho = 0;
+  bool captured = false;
+};
+
+static void batch_enqueue(afq_batch* b, int i, bool qr) {
+  afq_ctx* c = b->ctx[i];
+  DevState& s = c->s;
+  const int n = (int)b->ctx.size();
+  hipStream_t st = c->stream;
+  afq::launch_zero_counter(s.ncap, st);
+  afq::launch_fields(s, nullptr, b->seed + 7919ull * (uint64_t)i, 0, st);
+  if (s.prec_vhs == AFQ_PREC_FP64) afq::launch_vhs(s, st);
+  else afq::launch_vhs_lp(s, s.prec_vhs, st);
+  afq::launch_expmul_pre(s, st);
+  afq::launch_taylor(s, st);
+  afq::launch_expmul_post(s, st);
+  afq::launch_green(s, false, st);
+  afq::launch_weight(s, 0.0, true, st);
+  run_force_bias(s, st);
+  if (qr) afq::launch_qr(s, st);
+  if (s.GE > 0) {
+    if (s.prec_exchange == AFQ_PREC_FP64) afq::launch_exchange(s, nullptr, s.W, st);
+    else afq::launch_exchange_lp(s, nullptr, s.W, s.prec_exchange, st);
+  }
+  afq::launch_energy(s, nullptr, s.W, 0.0, st);
+  afq::launch_block_energy(s, b->d_par + 2 * n + i, b->halfw, b->d_tot + 2 * i, st);
+  afq::launch_comb(s, 0.0, st);
+  launch_check("batch step");
+}
+
+// all systems' step on their streams, forked from and joined back into the origin stream
+static void batch_record(afq_batch* b, bool qr) {
+  const int n = (int)b->ctx.size();
+  hip_check(hipEventRecord(b->fork, b->origin), "record");
+  for (int i = 0; i < n; ++i) {
+    afq_ctx* c = b->ctx[i];
+    DevState& s = c->s;
+    s.d_step = b->d_step;
+    s.d_eshift = b->d_par + i;
+    s.d_zeta = b->d_par + n + i;
+    s.graph_mode = true;
+    hip_check(hipStreamWaitEvent(c->stream, b->fork, 0), "wait");
+    try {
+      batch_enqueue(b, i, qr);
+    } catch (...) {
+      s.d_step = nullptr; s.d_eshift = nullptr; s.d_zeta = nullptr; s.graph_mode = false;
+      throw;
+    }
+    s.d_step = nullptr;
+    s.d_eshift = nullptr;
+    s.d_zeta = nullptr;
+    s.graph_mode = false;
+    hip_check(hipEventRecord(b->join[i], c->stream), "record");
+    hip_check(hipStreamWaitEvent(b->origin, b->join[i], 0), "wait");
+  }
+  afq::launch_step_inc(b->d_step, b->origin);
+}
+
+static void batch_free(afq_batch* b) {
+  for (int k = 0; k < 2; ++k) {
+    if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+  }
+  for (afq_ctx* c : b->ctx) afq_destroy(c);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->origin) (void)hipStreamDestroy(b->origin);
+  if (b->d_step) (void)hipFree(b->d_step);
+  if (b->d_par) (void)hipFree(b->d_par);
+  if (b->d_tot) (void)hipFree(b->d_tot);
+  if (b->h_par) (void)hipHostFree(b->h_par);
+  if (b->h_tot) (void)hipHostFree(b->h_tot);
+  delete b;
+}
+
+int afq_batch_create(int32_t nsys, const afq_ham* hams, const afq_trial* trials, const afq_params* params,
+                     afq_batch** out) {
+  if (nsys <= 0 || !hams || !trials || !params || !out) return fail(nullptr, AFQ_ERR_ARG, "bad batch arguments");
+  afq_batch* b = new afq_batch();
+  for (int i = 0; i < nsys; ++i) {
+    afq_ctx* c = nullptr;
+    const int rc = afq_create(&hams[i], &trials[i], &params[i], &c);
+    if (rc) {
+      batch_free(b);
+      return rc;
+    }
+    b->ctx.push_back(c);
+  }
+  try {
+    hip_check(hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking), "stream");
+    hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "event");
+    b->join.resize(nsys, nullptr);
+    for (int i = 0; i < nsys; ++i) hip_check(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "event");
+    b->d_step = dalloc<unsigned long long>(1);
+    b->d_par = dalloc<double>(3 * (size_t)nsys);
+    b->d_tot = dalloc<double>(2 * (size_t)nsys);
+    hip_check(hipHostMalloc((void**)&b->h_par, 3 * sizeof(double) * nsys, 0), "hipHostMalloc");
+    hip_check(hipHostMalloc((void**)&b->h_tot, 2 * sizeof(double) * nsys, 0), "hipHostMalloc");
+  } catch (const std::exception& e) {
+    batch_free(b);
+    return fail(nullptr, AFQ_ERR_DEVICE, e.what());
+  }
+  *out = b;
+  return AFQ_OK;
+}
+
+int afq_batch_destroy(afq_batch* b) {
+  if (!b) return AFQ_OK;
+  if (b->origin) (void)hipStreamSynchronize(b->origin);
+  batch_free(b);
+  return AFQ_OK;
+}
+
+int afq_batch_context(afq_batch* b, int32_t i, afq_ctx** c) {
+  if (!b || !c || i < 0 || i >= (int)b->ctx.size()) return fail(nullptr, AFQ_ERR_ARG, "bad system index");
+  *c = b->ctx[i];
+  return AFQ_OK;
+}
+
+int afq_batch_capture(afq_batch* b, uint64_t seed, int32_t ortho_steps, double clip_halfwidth, uint64_t first_step) {
+  if (!b || ortho_steps <= 0) return fail(nullptr, AFQ_ERR_ARG, "bad capture arguments");
+  for (afq_ctx* c : b->ctx)
+    if (!c->prepped || !c->have_walkers)
+      return fail(nullptr, AFQ_ERR_STATE, "every system needs afq_prep_prop and afq_set_walkers first");
+  try {
+    for (afq_ctx* c : b->ctx) hip_check(hipStreamSynchronize(c->stream), "sync");
+    b->seed = seed;
+    b->ortho = ortho_steps;
+    b->halfw = clip_halfwidth;
+    for (int k = 0; k < 2; ++k) {
+      if (b->exec[k]) { (void)hipGraphExecDestroy(b->exec[k]); b->exec[k] = nullptr; }
+      if (b->graph[k]) { (void)hipGraphDestroy(b->graph[k]); b->graph[k] = nullptr; }
+      hip_check(hipStreamBeginCapture(b->origin, hipStreamCaptureModeGlobal), "hipStreamBeginCapture");
+      try {
+        batch_record(b, k == 1);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->origin, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(b->origin, &b->graph[k]), "hipStreamEndCapture");
+      hip_check(hipGraphInstantiate(&b->exec[k], b->graph[k], nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    const unsigned long long s0 = first_step;
+    hip_check(hipMemcpyAsync(b->d_step, &s0, sizeof(s0), hipMemcpyHostToDevice, b->origin), "H2D");
+    hip_check(hipStreamSynchronize(b->origin), "sync");
+    b->captured = true;
+  } catch (const DevError& e) {
+    return fail(nullptr, e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(nullptr, AFQ_ERR_DEVICE, e.what());
+  }
+  return AFQ_OK;
+}
+
+// One step of every system.  eshift / zeta / center: per system; totals receives
+// {sum w, sum w clip(E_L)} per system (before the comb).  graph = 0 issues the same
+// kernels one launch at a time (no capture) for comparison.
+int afq_batch_step(afq_batch* b, int64_t step, const double* eshift, const double* zeta, const double* center,
+                   int32_t graph, double* totals) {
+  if (!b || !eshift || !zeta || !center || !totals) return fail(nullptr, AFQ_ERR_ARG, "bad step arguments");
+  if (!b->captured) return fail(nullptr, AFQ_ERR_STATE, "afq_batch_capture first");
+  const int n = (int)b->ctx.size();
+  try {
+    for (int i = 0; i < n; ++i) {
+      b->h_par[i] = eshift[i];
+      b->h_par[n + i] = zeta[i];
+      b->h_par[2 * n + i] = center[i];
+    }
+    hip_check(hipMemcpyAsync(b->d_par, b->h_par, 3 * sizeof(double) * n, hipMemcpyHostToDevice, b->origin), "H2D");
+    const bool qr = step % b->ortho == 0;
+    if (graph) {
+      hip_check(hipGraphLaunch(b->exec[qr ? 1 : 0], b->origin), "hipGraphLaunch");
+    } else {
+      batch_record(b, qr);
+    }
+    hip_check(hipMemcpyAsync(b->h_tot, b->d_tot, 2 * sizeof(double) * n, hipMemcpyDeviceToHost, b->origin), "D2H");
+    hip_check(hipStreamSynchronize(b->origin), "sync");
+    for (int i = 0; i < 2 * n; ++i) totals[i] = b->h_tot[i];
+  } catch (const DevError& e) {
+    return fail(nullptr, e.code, e.what());
+  } catch (const std::exception& e) {
+    return fail(nullptr, AFQ_ERR_DEVICE, e.what());
+  }
+  return AFQ_OK;
+}

@@ -83,6 +83,13 @@ struct DevState {
   unsigned short* Tb_hi = nullptr; unsigned short* Tb_lo = nullptr;  // [w][re/im][slot][j][Np]
   int prec_fb = AFQ_PREC_FP64, prec_vhs = AFQ_PREC_FP64;
   int prec_prop = AFQ_PREC_FP64;   // Taylor products V t
+  // replayed (hipGraph) steps: step counter, energy shift and comb uniform live in
+  // device memory, and the comb copies back instead of swapping buffers
+  const unsigned long long* d_step = nullptr;
+  const double* d_eshift = nullptr;
+  const double* d_zeta = nullptr;
+  bool graph_mode = false;
+  int taylor_variant = 0;   // 0: by shape, 1: k_vmul (64 x 128 tiles), 2: k_taylor (128 x 128 complex)
   int Gp = 0;               // G padded to a multiple of 8
   unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
   unsigned short* Fb_hi = nullptr; unsigned short* Fb_lo = nullptr;  // fields, [w][re/im][Gp]
@@ -116,6 +123,8 @@ void launch_ek(const DevState& s, const int* idx, int n, double* ek, float* yout
 void launch_predict(const DevState& s, const Mlp& m, const float* X, int n, double* S, hipStream_t st);
 void launch_ej(const DevState& s, double* ej, hipStream_t st);
 void launch_sub(double* a, const double* b, int n, hipStream_t st);  // a -= b
+void launch_block_energy(const DevState& s, const double* center, double halfw, double* out, hipStream_t st);
+void launch_step_inc(unsigned long long* step, hipStream_t st);
 void launch_energy_cv(const DevState& s, const double* S, const double* ek, const float* pinv, double delta,
                       double soff, const double* center, double halfw, double* dres, hipStream_t st);
 

@@ -93,10 +93,11 @@ __device__ inline double philox_normal(uint64_t seed, int w, uint64_t step, int 
 __global__ __launch_bounds__(256) void k_fields(const double* fb, const double* mf, const double* draws,
                                                 const double* weights, double* fields, double* sft,
                                                 int G, double dt, int ref_compat, uint64_t seed,
-                                                uint64_t step) {
+                                                uint64_t step, const unsigned long long* step_dev) {
   __shared__ double red[16];
   const int w = blockIdx.x;
   if (weights[w] <= 1e-8) return;
+  if (step_dev) step = *step_dev;  // replayed graphs read the step counter from memory
   const double sq = sqrt(dt);
   double sh_re = 0, sh_im = 0, ft_re = 0, ft_im = 0;
   for (int g = threadIdx.x; g < G; g += blockDim.x) {
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(256) void k_fields(const double* fb, const double* 
 
 void launch_fields(const DevState& s, const double* draws, uint64_t seed, uint64_t step, hipStream_t st) {
   hipLaunchKernelGGL(k_fields, dim3(s.W), dim3(256), 0, st, s.fb, s.mf, draws, s.weights, s.fields,
-                     s.sft, s.G, s.dt, (int)s.ref_compat, seed, step);
+                     s.sft, s.G, s.dt, (int)s.ref_compat, seed, step, s.d_step);
 }
 
 // ------------------------------------------------------------------ VHS
@@ -532,7 +533,7 @@ void launch_taylor(const DevState& s, hipStream_t st) {
   const double sq = sqrt(s.dt);
   double* src = s.t0;
   double* dst = s.t1;
-  const bool wide = n2a + n2b > 128;
+  const bool wide = s.taylor_variant == 0 ? n2a + n2b > 128 : s.taylor_variant == 2;
   const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
   for (int p = 1; p <= 5; ++p) {
     if (s.prec_prop != AFQ_PREC_FP64) {
@@ -715,9 +716,11 @@ void launch_green(const DevState& s, bool all, hipStream_t st) {
 // (DQMC/MixedEstimator.cpp:461-468), with log overlaps.
 __global__ void k_weight(double* weights, double* factors, double* logO, const double* logdet,
                          const double* sft, int W, int nslots, int m0, int m1, double dt, double eshift,
-                         double pc_re, double cap, unsigned long long* ncap, int update) {
+                         double pc_re, double cap, unsigned long long* ncap, int update,
+                         const double* eshift_dev) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
+  if (eshift_dev) eshift = *eshift_dev;
   double f = 1.0;
   if (weights[w] > 1e-8) {
     double lo_re = m0 * logdet[2 * (int64_t)w * nslots], lo_im = m0 * logdet[2 * (int64_t)w * nslots + 1];
@@ -745,7 +748,7 @@ __global__ void k_weight(double* weights, double* factors, double* logO, const d
 void launch_weight(const DevState& s, double eshift, bool update, hipStream_t st) {
   hipLaunchKernelGGL(k_weight, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.weights, s.factors, s.logO,
                      s.logdet, s.sft, s.W, s.nslots, s.slot[0].mult, s.nslots > 1 ? s.slot[1].mult : 0,
-                     s.dt, eshift, s.prop_const_re, s.weight_cap, s.ncap, (int)update);
+                     s.dt, eshift, s.prop_const_re, s.weight_cap, s.ncap, (int)update, s.d_eshift);
 }
 
 __global__ void k_logO(double* logO, const double* logdet, int W, int nslots, int m0, int m1) {
@@ -1587,9 +1590,11 @@ __global__ void k_cumsum(const double* w, double* cum, int W) {
     for (int i = 0; i < W; ++i) { c = (i == 0) ? w[0] : c + w[i]; cum[i] = c; }
   }
 }
-__global__ void k_comb_index(const double* cum, int* idx, double* weights, int W, double zeta) {
+__global__ void k_comb_index(const double* cum, int* idx, double* weights, int W, double zeta,
+                             const double* zeta_dev) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
+  if (zeta_dev) zeta = *zeta_dev;
   const double tot = cum[W - 1];
   const double z = (w + zeta) / W;
   const double target = z * tot;
@@ -1612,13 +1617,56 @@ __global__ void k_gather(const int* idx, const double* src, double* dst, int64_t
     d[i] = s[i];
 }
 
+__global__ void k_copy_calib(const double* __restrict__ a, double* __restrict__ b, int64_t n);
+
+// Block totals of one population: out = {sum w, sum w clip(E_L)} over the live
+// walkers, the clip to center -+ halfw of DQMC/MixedEstimator.cpp:525-530 (none
+// when center is null).
+__global__ __launch_bounds__(256) void k_block_energy(const double* eloc, const double* weights, int W,
+                                                      const double* center, double halfw, double* out) {
+  __shared__ double red[16];
+  double sw = 0.0, swe = 0.0;
+  const double c0 = center ? *center : 0.0;
+  for (int w = threadIdx.x; w < W; w += blockDim.x) {
+    const double wt = weights[w];
+    if (wt != 0.0) {
+      double e = eloc[w];
+      if (center) e = fmin(fmax(e, c0 - halfw), c0 + halfw);
+      sw += wt;
+      swe += wt * e;
+    }
+  }
+  sw = block_sum(sw, red);
+  swe = block_sum(swe, red);
+  if (threadIdx.x == 0) { out[0] = sw; out[1] = swe; }
+}
+void launch_block_energy(const DevState& s, const double* center, double halfw, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_block_energy, dim3(1), dim3(256), 0, st, s.eloc, s.weights, s.W, center, halfw, out);
+}
+
+__global__ void k_step_inc(unsigned long long* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1ull;
+}
+void launch_step_inc(unsigned long long* step, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, st, step);
+}
+
 void launch_comb(DevState& s, double zeta, hipStream_t st) {
   hipLaunchKernelGGL(k_cumsum, dim3(1), dim3(64), 0, st, s.weights, s.cum, s.W);
-  hipLaunchKernelGGL(k_comb_index, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.cum, s.idx, s.weights, s.W, zeta);
+  hipLaunchKernelGGL(k_comb_index, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.cum, s.idx, s.weights, s.W, zeta,
+                     s.d_zeta);
+  // eager calls swap the buffers; a captured step copies back so every replay sees
+  // the walkers where the graph's nodes expect them
   auto gather = [&](double*& a, double*& b, int64_t per) {
     const unsigned bx = (unsigned)std::min<int64_t>(64, (per + 255) / 256);
     hipLaunchKernelGGL(k_gather, dim3(bx, s.W), dim3(256), 0, st, s.idx, a, b, per);
-    std::swap(a, b);
+    if (s.graph_mode) {
+      const int64_t n = per * s.W;
+      hipLaunchKernelGGL(k_copy_calib, dim3((unsigned)std::min<int64_t>(cdivu(n, 256), 65535)), dim3(256), 0, st,
+                         b, a, n);
+    } else {
+      std::swap(a, b);
+    }
   };
   gather(s.psi, s.psi2, 2 * s.ms);
   gather(s.theta, s.theta2, 2 * s.Ktot);
