@@ -258,6 +258,81 @@ def micro(kind, workload, nwalk=1024, calls=8):
     print(json.dumps({"micro": kind, "workload": workload, "nwalk": nwalk, "ms_per_measure": ms}))
 
 
+def bench_batched(a):
+    """The uneven multi-system workload: 64 systems, N ~ U{20..120}, n = round(N/5),
+    G = 4N, 256 walkers each; every step of every system is one hipGraph launch."""
+    from vmc_amd.synthetic import batched_systems
+    t0 = time.time()
+    hams = batched_systems()  # worker processes: before any GPU call in this process
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
+    log("[bench] %d systems built in %.1fs" % (len(hams), time.time() - t0))
+    from vmc_amd.abi import AFQ_PREC_BF16X3
+    from vmc_amd.afqmc import Batch
+    W, dt, ortho = 256, 0.005, 5
+    half = np.sqrt(2.0 / dt)
+    nw_total = W * len(hams)
+    flops = sum(flops_per_walker_step(dict(norb=h.norb, nchol=h.nchol, nalpha=h.nalpha, nbeta=h.nbeta,
+                                           walker_rhf=False)) for h in hams) * W
+    paths, first = {}, {}
+    for name, mixed, graph in (("fp64_eager", False, False), ("fp64_graph", False, True),
+                               ("mixed_graph", True, True)):
+        b = Batch(hams, False, W)
+        et = np.zeros(len(hams))
+        for i, (c, h) in enumerate(zip(b.ctx, hams)):
+            c.prep_prop(h.phi_up, h.phi_dn, dt)
+            o = c.ham_and_overlap(h.serial_det(h.phi_up, h.phi_dn)[None, :], h.nchol)[0]
+            et[i] = (o[0] / o[1]).real
+            c.set_walkers(np.tile(h.serial_det(h.phi_up, h.phi_dn), (W, 1)))
+            if mixed:
+                c.set_precision(prec_fb=AFQ_PREC_BF16X3, prec_vhs=AFQ_PREC_BF16X3, prec_exchange=AFQ_PREC_BF16X3)
+                c.set_prop_precision(AFQ_PREC_BF16X3)
+        tune = b.autotune()
+        b.capture(20261202, ortho, half, first_step=1)
+        eshift, eest, center = et.copy(), et.copy(), et.copy()
+        rng = np.random.default_rng(5)
+        total = a.warmup + a.steps
+        tstart = None
+        for step in range(1, total + 1):
+            if step == a.warmup + 1:
+                tstart = time.time()
+            tot = b.step(step, eshift, rng.uniform(size=len(hams)), center, graph=graph)
+            eb = tot[:, 1] / tot[:, 0]
+            if step == 1:
+                first[name] = eb.copy()
+            eest = 0.9 * eest + 0.1 * eb
+            eshift = eest - 0.1 * np.log(tot[:, 0] / W) / dt
+            center = eest
+        wall = time.time() - tstart
+        b.close()
+        ms = 1e3 * wall / a.steps
+        paths[name] = dict(walker_steps_per_s=nw_total * a.steps / wall, ms_per_step=ms,
+                           step_tflops_algorithmic=flops * a.steps / wall / 1e12,
+                           taylor_tiles={"64x128": sum(1 for ch, _ in tune if ch == 1),
+                                         "128x128c": sum(1 for ch, _ in tune if ch == 2)})
+        log("[bench] %s: %.1f walker-steps/s, %.2f ms/step" % (name, paths[name]["walker_steps_per_s"], ms))
+    rel = float(np.max(np.abs(first["mixed_graph"] - first["fp64_graph"]) / np.abs(first["fp64_graph"])))
+    best = paths["mixed_graph"]
+    ach = best["step_tflops_algorithmic"]
+    out = {
+        "metric": METRIC, "value": best["walker_steps_per_s"], "unit": "walker-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+bf16", "data": "synthetic",
+        "config": {"workload": "%d systems, N ~ U{20..120} (sum %d), n = round(N/5), G = 4N, UHF trials, %d walkers "
+                               "each, dt=%g, E_L every step, QR every %d, one hipGraph launch per step"
+                               % (len(hams), sum(h.norb for h in hams), W, dt, ortho),
+                   "name": "c2", "path": "mixed_graph"},
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAKS["fp64"], "unit": "TFLOP/s",
+                     "frac": ach / PEAKS["fp64"], "traffic": None, "kernel": "whole step, all systems",
+                     "note": "algorithmic FLOPs of every system's step over the FP64 MFMA peak"},
+        "cpu_baseline": None,
+        "paths": paths,
+        "speedup_vs_fp64_graph": best["walker_steps_per_s"] / paths["fp64_graph"]["walker_steps_per_s"],
+        "graph_speedup_fp64": paths["fp64_graph"]["walker_steps_per_s"] / paths["fp64_eager"]["walker_steps_per_s"],
+        "first_step_energy_rel_diff_mixed_vs_fp64": rel,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +361,12 @@ def main():
     a = ap.parse_args()
     if a.micro:
         micro(a.micro, a.workload)
+        return
+    if a.workload == "c2":
+        if a.gpus != 1:
+            print("bench.py: this build runs on one GPU; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+            sys.exit(2)
+        bench_batched(a)
         return
     if a.gpus != 1:
         print("bench.py: this build runs one GPU holding one rank's walker population; "

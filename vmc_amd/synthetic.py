@@ -137,12 +137,33 @@ def workload(name, seed=DEFAULT_SEED):
     return ham, spec
 
 
-def batched_systems(nsys=64, seed=DEFAULT_SEED, nmin=20, nmax=120):
-    """The uneven multi-system workload: N ~ U{nmin..nmax}, n = round(N/5), G = 4N."""
+def _one_thread():
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except ImportError:
+        pass
+
+
+def _batched_member(args):
+    N, n, seed = args
+    return make_system(N, n, n, 4 * N, seed=seed, trial="uhf")
+
+
+def batched_systems(nsys=64, seed=DEFAULT_SEED, nmin=20, nmax=120, workers=None):
+    """The uneven multi-system workload: N ~ U{nmin..nmax}, n = round(N/5), G = 4N.
+    The systems are independent, so they are generated by a pool of single-threaded
+    worker processes (at most 16)."""
+    import os
     rng = np.random.default_rng(seed + 7)
-    out = []
+    specs = []
     for s in range(nsys):
         N = int(rng.integers(nmin, nmax + 1))
         n = max(1, int(round(N / 5)))
-        out.append(make_system(N, n, n, 4 * N, seed=seed + 100 + s, trial="uhf"))
-    return out
+        specs.append((N, n, seed + 100 + s))
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1:
+        return [_batched_member(x) for x in specs]
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers, initializer=_one_thread) as ex:
+        return list(ex.map(_batched_member, specs))
