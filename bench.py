@@ -349,6 +349,8 @@ def main():
                          "always measured exactly")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-surrogate path")
+    ap.add_argument("--surrogate-prec", choices=["bf16", "bf16x3"], default="bf16",
+                    help="precision of the exchange surrogate level of the control variate")
     ap.add_argument("--fp64-propagation", action="store_true",
                     help="mixed path: keep the force bias, VHS and Taylor products in FP64 "
                          "(default: split BF16)")
@@ -400,15 +402,16 @@ def main():
         xcv = run_path(ham, spec, a.warmup + a.steps, a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
                        prec_fb=pp, prec_vhs=pp, prec_prop=pp, prec_exchange=AFQ_PREC_FP64,
                        surrogate_kind=AFQ_SURR_EXCHANGE,
-                       surrogate_prec=AFQ_PREC_BF16, subset_inv=a.subset_inv, risk_frac=0.0, shadow_exact=True)
+                       surrogate_prec=AFQ_PREC_BF16X3 if a.surrogate_prec == "bf16x3" else AFQ_PREC_BF16,
+                       subset_inv=a.subset_inv, risk_frac=0.0, shadow_exact=True)
         nx = W * xcv["live_frac"] / a.subset_inv
-        xcv.update(exchange_prec="fp64", surrogate_prec="bf16", n_exact=nx,
+        xcv.update(exchange_prec="fp64", surrogate_prec=a.surrogate_prec, n_exact=nx,
                    fb_vhs_prec="fp64" if a.fp64_propagation else "bf16x3")
         xcv.update(cv_summary(xcv, fp64))
         paths["mixed_exchange_cv"] = xcv
-        log("[bench] mixed (bf16 exchange level + fp64 1/%d subset): %.1f walker-steps/s, %.1f ms/step, "
+        log("[bench] mixed (%s exchange level + fp64 1/%d subset): %.1f walker-steps/s, %.1f ms/step, "
             "speedup %.2fx, shift %.4f +- %s mHa, err/sqrt(s) ratio %s, live %.3f"
-            % (a.subset_inv, xcv["walker_steps_per_s"], xcv["ms_per_step"], xcv["speedup_vs_fp64"],
+            % (a.surrogate_prec, a.subset_inv, xcv["walker_steps_per_s"], xcv["ms_per_step"], xcv["speedup_vs_fp64"],
                xcv["energy_shift_mha"], xcv["energy_shift_sem_mha"], xcv["err_per_sqrt_s_ratio"],
                xcv["live_frac"]))
         best_name = "mixed_exchange_cv"

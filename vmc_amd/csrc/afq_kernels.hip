@@ -1296,7 +1296,7 @@ __global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u
                                                           const int* sel, double* Xp, int Np, int GE, int nw, int W,
                                                           int G, Slot s0, Slot s1, int ctot, int ngt, int gridx,
                                                           int gridy, int gridz) {
-  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, (NS == 1 ? 64 : 32)>;
+  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, 64>;
   constexpr int LDF = XC_BN + 1;
   constexpr int FT_BYTES = (XC_BN / 2) * LDF * 4;
   constexpr int LDSB = Eng::LDS_BYTES > FT_BYTES ? Eng::LDS_BYTES : FT_BYTES;
