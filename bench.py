@@ -158,13 +158,22 @@ def stage_rooflines(spec, path, n_exact):
         add("vhs", "k_vhs (FP64 MFMA)", W * 4.0 * T * G, 8.0 * G * T + W * 16.0 * (G + N * N), 1, "fp64",
             symbol="k_vhs")
     else:
-        add("taylor", "k_taylor_bf16 (5 complex V t products, 3-pass split BF16 MFMA)", W * 40.0 * N * N * ncol,
-            tay_bytes, 5, "bf16", symbol="k_taylor_bf16", passes=3)
+        # Horner steps on split operands: V (hi, lo: 8 Nq N bytes), u in, t0 and u out
+        # (4 planes of bf16: 8 ncol Nq bytes each); the last step adds the FP64 t0 and
+        # writes FP64
+        Nq = (N + 15) // 16 * 16
+        vb, ub = 8.0 * Nq * N, 8.0 * ncol * Nq
+        tay_split = W * (4 * (vb + 3 * ub) + (vb + ub + 32.0 * ms))
+        add("taylor", "k_taylor_h (5 Horner steps u = t0 + V u / p, 3-pass split BF16 MFMA)",
+            W * 40.0 * N * N * ncol, tay_split, 5, "bf16", symbol="void k_taylor_h<%d, %d>" % (
+                5 if -(-N // 160) * 160 < -(-N // 128) * 128 else 4, 1 if ncol <= 32 else 2 if ncol <= 64 else 4),
+            passes=3)
         add("force_bias", "k_theta_split + k_force_bias_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * ncol * N * G,
             4.0 * G * ncol * Np + W * (16.0 * ms + 8.0 * ncol * Np + 16.0 * G), 1, "bf16",
             symbol="void k_force_bias_bf16<2>", passes=3)
         add("vhs", "k_fields_split + k_vhs_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * T * G,
-            4.0 * G * T + W * (16.0 * G + 16.0 * N * N), 1, "bf16", symbol="void k_vhs_bf16<2>", passes=3)
+            4.0 * G * T + W * (24.0 * G + 8.0 * ((N + 15) // 16 * 16) * N), 1, "bf16",
+            symbol="void k_vhs_bf16<2>", passes=3)
     add("onebody", "k_expmul (E psi, FP64 MFMA)", W * 8.0 * N * N * ncol, W * 3 * 16.0 * ms, 2 * len(nsp), "fp64",
         symbol="k_expmul")
     add("green", "k_green (FP64 MFMA + Gauss-Jordan)",

@@ -256,12 +256,13 @@ int afq_set_energy_clip(afq_ctx* ctx, const double* center, double halfwidth);
 int afq_cv_stats(afq_ctx* ctx, double* out6);
 
 /* ---------------------------------------------------------------- tuning
- * Tile-shape autotuner (no counterpart in the reference): times the alternative
- * Taylor-product kernels (64 x 128 real tiles; 128 x 128 complex tiles) on the
- * context's current walkers and keeps the faster.  choice (nullable) receives 1 or
- * 2 (0: nothing to choose in the current precision); ms2 (nullable) the two times. */
+ * Tile-shape autotuner (no counterpart in the reference): times the two Taylor-product
+ * tile shapes of the context's propagation precision on its current walkers and keeps
+ * the faster.  FP64: 1 = 64 x 128 real tiles, 2 = 128 x 128 complex tiles; split BF16:
+ * 1 = 128-orbital row tiles, 2 = 160-orbital row tiles.  choice (nullable) receives 1
+ * or 2; ms2 (nullable) the two times (ms per Taylor series). */
 int afq_autotune(afq_ctx* ctx, int32_t* choice, double* ms2);
-/* 0: by shape (default), 1: 64 x 128 real tiles, 2: 128 x 128 complex tiles */
+/* 0: by shape (default), 1 or 2: the tile shapes named at afq_autotune */
 int afq_set_taylor_variant(afq_ctx* ctx, int32_t variant);
 
 /* ---------------------------------------------------------------- batched systems

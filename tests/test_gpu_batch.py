@@ -115,6 +115,16 @@ def test_tile_variants_agree(setup):
     assert rel(a, b) < 1e-10
 
 
+def test_split_tile_variants_agree(setup):
+    """The split-BF16 Taylor series with 128- and 160-orbital row tiles: the K loop
+    and every element's arithmetic are the same, so the totals agree to rounding."""
+    hams, eshift, center, zeta = setup
+    a = run_batch(hams, eshift, center, zeta, mixed=True, variant=1)
+    b = run_batch(hams, eshift, center, zeta, mixed=True, variant=2)
+    print("split variants", rel(a, b))
+    assert rel(a, b) < 1e-10
+
+
 def test_autotune_picks_a_variant(setup):
     hams, eshift, center, zeta = setup
     b = Batch(hams[-2:], False, W)
@@ -124,6 +134,16 @@ def test_autotune_picks_a_variant(setup):
     res = b.autotune()
     b.close()
     print("autotune", res)
+    assert all(ch in (1, 2) and np.all(ms > 0) for ch, ms in res)
+    b = Batch(hams[-2:], False, W)
+    for c, h in zip(b.ctx, hams[-2:]):
+        prepare(c, h)
+        c.set_precision(prec_fb=AFQ_PREC_BF16X3, prec_vhs=AFQ_PREC_BF16X3, prec_exchange=AFQ_PREC_BF16X3)
+        c.set_prop_precision(AFQ_PREC_BF16X3)
+        c.propagate(None, eshift[0], seed=1, step=1)
+    res = b.autotune()
+    b.close()
+    print("autotune split", res)
     assert all(ch in (1, 2) and np.all(ms > 0) for ch, ms in res)
 
 

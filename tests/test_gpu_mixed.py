@@ -51,13 +51,13 @@ def test_force_bias_bf16x3_vs_fp64(name):
     assert err.max() < 2e-5
 
 
-@pytest.mark.parametrize("prop", [False, True])
-@pytest.mark.parametrize("name", ["uhf_uhf", "c0_rhf", "uhf_n60", "c1_small"])
+@pytest.mark.parametrize("prop", [False, True, "vhs64"])
+@pytest.mark.parametrize("name", ["uhf_uhf", "rhf_uhf", "c0_rhf", "uhf_33_31", "uhf_n60", "c1_small"])
 def test_mixed_step_vs_fp64(name, prop):
     """Three steps from the same walkers with the same draws: force bias, VHS and
     exchange (and with prop, the Taylor products) in BF16x3 against all-FP64.  The
     propagated walkers, weights and local energies stay within 1e-5 (relative) of
-    the FP64 path."""
+    the FP64 path.  "vhs64": an FP64 VHS feeding the split-BF16 Taylor series."""
     W = 6
     h, wrhf, ctx, orc = build(name, W)
     h2, _, ctx2, _ = build(name, W)
@@ -66,7 +66,9 @@ def test_mixed_step_vs_fp64(name, prop):
     o = ctx.ham_and_overlap(h.serial_det(*ctx.ref_dets)[None, :], h.nchol)[0]
     eshift = (o[0] / o[1]).real
     ctx.set_walkers(dets)
-    ctx2.set_precision(prec_fb=AFQ_PREC_BF16X3, prec_vhs=AFQ_PREC_BF16X3, prec_exchange=AFQ_PREC_BF16X3)
+    ctx2.set_precision(prec_fb=AFQ_PREC_BF16X3,
+                       prec_vhs=AFQ_PREC_FP64 if prop == "vhs64" else AFQ_PREC_BF16X3,
+                       prec_exchange=AFQ_PREC_BF16X3)
     if prop:
         ctx2.set_prop_precision(AFQ_PREC_BF16X3)
     ctx2.set_walkers(dets)

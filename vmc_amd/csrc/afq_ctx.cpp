@@ -179,7 +179,8 @@ void free_all(DevState& s) {
     if (p) (void)hipFree(p);
   if (s.idx) (void)hipFree(s.idx);
   if (s.ncap) (void)hipFree(s.ncap);
-  unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo, s.Lt_hi, s.Lt_lo, s.Fb_hi, s.Fb_lo};
+  unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo, s.Lt_hi, s.Lt_lo, s.Fb_hi, s.Fb_lo,
+                          s.Vh, s.Vl, s.Ub[0], s.Ub[1], s.Ub[2]};
   for (unsigned short* p : hp)
     if (p) (void)hipFree(p);
 }
@@ -209,6 +210,23 @@ void ensure_vhs_lowprec(DevState& s, hipStream_t st) {
   launch_check("lt_split");
 }
 
+// split-BF16 Taylor operands (allocated on first use), zero so that the orbital pads
+// of every chunk stay zero
+void ensure_prop_lowprec(DevState& s, hipStream_t st) {
+  if (s.Vh) return;
+  s.Nq = (s.N + 15) / 16 * 16;
+  const int ncolc = s.det_n[0] + (s.ndets_prop > 1 ? s.det_n[1] : 0);
+  const int64_t nv = (int64_t)s.W * s.Nq * s.N * 2;       // u16 per V plane (chunks x N x 32)
+  const int64_t nu = (int64_t)s.W * 4 * ncolc * s.Nq;     // u16 per u buffer
+  s.Vh = dalloc<unsigned short>(nv);
+  s.Vl = dalloc<unsigned short>(nv);
+  for (int k = 0; k < 3; ++k) s.Ub[k] = dalloc<unsigned short>(nu);
+  afq::launch_fill(reinterpret_cast<double*>(s.Vh), nv / 4, 0.0, st);
+  afq::launch_fill(reinterpret_cast<double*>(s.Vl), nv / 4, 0.0, st);
+  for (int k = 0; k < 3; ++k) afq::launch_fill(reinterpret_cast<double*>(s.Ub[k]), nu / 4, 0.0, st);
+  launch_check("prop split buffers");
+}
+
 // force bias of the current theta in the context's force-bias precision
 void run_force_bias(const DevState& s, hipStream_t st) {
   if (s.prec_fb == AFQ_PREC_FP64) afq::launch_force_bias(s, st);
@@ -228,6 +246,7 @@ void free_temporary(const DevState& t) {
   z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
   z.Rb_hi = z.Rb_lo = z.Tb_hi = z.Tb_lo = nullptr;
   z.Lt_hi = z.Lt_lo = z.Fb_hi = z.Fb_lo = nullptr;
+  z.Vh = z.Vl = z.Ub[0] = z.Ub[1] = z.Ub[2] = nullptr;
   free_all(z);
 }
 
@@ -593,8 +612,11 @@ int afq_set_prop_precision(afq_ctx* c, int32_t prec_prop) {
     return fail(c, AFQ_ERR_UNSUPPORTED, "propagation precision must be FP64 or BF16X3");
   if (prec_prop != AFQ_PREC_FP64 && c->s.ref_compat)
     return fail(c, AFQ_ERR_UNSUPPORTED, "ref_compat needs the FP64 propagation");
+  AFQ_GUARD_BEGIN
+  if (prec_prop != AFQ_PREC_FP64) ensure_prop_lowprec(c->s, c->stream);
   c->s.prec_prop = prec_prop;
   return AFQ_OK;
+  AFQ_GUARD_END(c)
 }
 
 int afq_set_nchol_ene(afq_ctx* c, int32_t n) {
@@ -1453,10 +1475,6 @@ int afq_autotune(afq_ctx* c, int32_t* choice, double* ms_out) {
   if (!c->prepped || !c->have_walkers) return fail(c, AFQ_ERR_STATE, "afq_prep_prop and afq_set_walkers first");
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
-  if (s.prec_prop != AFQ_PREC_FP64) {  // one split-BF16 kernel: nothing to choose
-    if (choice) *choice = 0;
-    return AFQ_OK;
-  }
   hipEvent_t a, b;
   hip_check(hipEventCreate(&a), "event");
   hip_check(hipEventCreate(&b), "event");

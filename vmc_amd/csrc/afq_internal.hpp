@@ -83,13 +83,23 @@ struct DevState {
   unsigned short* Tb_hi = nullptr; unsigned short* Tb_lo = nullptr;  // [w][re/im][slot][j][Np]
   int prec_fb = AFQ_PREC_FP64, prec_vhs = AFQ_PREC_FP64;
   int prec_prop = AFQ_PREC_FP64;   // Taylor products V t
+  // split-BF16 Taylor operands (allocated when prec_prop is BF16X3); orbitals come in
+  // chunks of 16 (Nq = N padded to 16, pads stay zero):
+  //   Vh/Vl  [w][chunk][i][32]: hi/lo of [-P_im | P_re] over the chunk's 16 orbitals
+  //   Ub[0], Ub[1] (Horner terms) and Ub[2] (E psi): [w][plane][chunk][q][16], planes
+  //          re_hi, re_lo, im_hi, im_lo; q runs over the na + nb propagated columns
+  int Nq = 0;
+  unsigned short* Vh = nullptr; unsigned short* Vl = nullptr;
+  unsigned short* Ub[3] = {nullptr, nullptr, nullptr};
   // replayed (hipGraph) steps: step counter, energy shift and comb uniform live in
   // device memory, and the comb copies back instead of swapping buffers
   const unsigned long long* d_step = nullptr;
   const double* d_eshift = nullptr;
   const double* d_zeta = nullptr;
   bool graph_mode = false;
-  int taylor_variant = 0;   // 0: by shape, 1: k_vmul (64 x 128 tiles), 2: k_taylor (128 x 128 complex)
+  // Taylor tile shape: 0 by shape; FP64: 1 k_vmul (64 x 128 tiles), 2 k_taylor (128 x 128
+  // complex); split BF16: 1 128-row tiles, 2 160-row tiles
+  int taylor_variant = 0;
   int Gp = 0;               // G padded to a multiple of 8
   unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
   unsigned short* Fb_hi = nullptr; unsigned short* Fb_lo = nullptr;  // fields, [w][re/im][Gp]
@@ -143,6 +153,7 @@ void launch_qr(const DevState& s, hipStream_t st);
 void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st);
 void launch_energy(const DevState& s, const int* sel, int nsel, double delta, hipStream_t st);
 void launch_comb(DevState& s, double zeta, hipStream_t st);
+void launch_v_split(const DevState& s, hipStream_t st);
 void launch_copy_det0_to_det1(const DevState& s, double* dst, const double* src, hipStream_t st);
 
 void launch_pack_chol(const double* L, double* Lp, int N, int G, int64_t T, bool round_f32, hipStream_t st);

@@ -203,11 +203,13 @@ struct DetColsB {  // B(k, q) = src[w][det d][k][cc], q = w*2n + cc
   }
 };
 constexpr int EX_BM = 64, EX_BN = 128;
-// mode 0: dst1 = dst2 = E src (start of the Taylor series); mode 1: dst1 = E src for live walkers
+// mode 0: dst1 = dst2 = E src (start of the Taylor series); mode 1: dst1 = E src for live walkers;
+// mode 2: dst1 = E src and its split-BF16 copy in t0b (columns from coloff of the ncolc
+// propagated columns, layout [w][re_hi, re_lo, im_hi, im_lo][orbital chunk of 16][q][16])
 __global__ __launch_bounds__(256) void k_expmul(const double* E, const double* src, double* dst1,
                                                 double* dst2, const double* weights, int N, int n,
                                                 int W, int64_t wstride, int64_t doff, int mode, int gridx,
-                                                int gridy) {
+                                                int gridy, u16* t0b, int coloff, int ncolc, int Nq) {
   using Eng = MmaF64<EX_BM, EX_BN, 2, 2, false, true>;
   __shared__ double lds[Eng::LDS_DOUBLES];
   const TileId tl = tile_id(gridx, gridy, 1, gridy);
@@ -223,6 +225,16 @@ __global__ __launch_bounds__(256) void k_expmul(const double* E, const double* s
     const int64_t a = (int64_t)w * wstride + doff + (int64_t)i * n2 + cc;
     dst1[a] = v;
     if (mode == 0) dst2[a] = v;
+    if (mode == 2) {
+      const int64_t plane = (int64_t)ncolc * Nq;
+      const int comp = cc & 1;
+      const int64_t b = (int64_t)w * 4 * plane + 2 * comp * plane + (int64_t)(i >> 4) * ncolc * 16 +
+                        (int64_t)(coloff + (cc >> 1)) * 16 + (i & 15);
+      u16 h, l;
+      split_bf16(v, h, l);
+      t0b[b] = h;
+      t0b[b + plane] = l;
+    }
   });
 }
 
@@ -395,135 +407,231 @@ __global__ __launch_bounds__(512) void k_taylor(const double* P, const double* t
       }
 }
 
-// The same product with split-BF16 operands (x = hi + lo, three MFMA passes
-// hi*hi + hi*lo + lo*hi, FP32 accumulation): P and t are read in FP64 and split
-// while staging into LDS.  A K stage is 16 orbitals = 32 real K rows, one
-// v_mfma_f32_16x16x32_bf16 deep; LDS keeps A as [row][k] and B as [col][k].
-constexpr int TB_KC = 16, TB_K = 2 * TB_KC, TB_LDK = TB_K + 8;
-__global__ __launch_bounds__(512) void k_taylor_bf16(const double* P, const double* tsrc, double* tdst, double* acc,
-                                                     const double* weights, int N, int na, int nb, int64_t offb,
-                                                     int64_t wstride, double scale, int gridx, int gridy, int W) {
-  __shared__ __attribute__((aligned(16))) u16 As[2][TY_BM * TB_LDK];
-  __shared__ __attribute__((aligned(16))) u16 Bs[2][TY_BN * TB_LDK];
+// Split-BF16 Taylor series by Horner's rule: u <- t0 + (V u) sqrt(dt)/p for p = 5..1,
+// starting from u = t0 = E psi, so that the last step leaves sum_{k<=5} V^k/k! t0.
+// V = i sqrt(dt) P enters as the real matrix [A_re | A_im] = [-P_im | P_re] against
+// [[u_re, u_im], [-u_im, u_re]], three MFMA passes hi*hi + hi*lo + lo*hi (x = hi + lo)
+// with FP32 accumulation.  All operands arrive already split (VHS epilogue, k_expmul,
+// previous Horner step) in orbital chunks of 16, so a K stage is one chunk: 16 orbitals
+// = 32 real K = one v_mfma_f32_16x16x32_bf16, and its A and B slices are contiguous.
+// Intermediate u are stored split; the last step adds the FP64 t0 and writes FP64.
+// Tile: 32*MI orbital rows x 32*NI complex columns, 8 waves as 2 (rows) x 4 (columns).
+constexpr int TH_LDK = 40;
+template <int MI, int NI>
+__global__ __launch_bounds__(512) void k_taylor_h(const u16* Vh, const u16* Vl, const u16* Ui, u16* Uo,
+                                                  const u16* T0b, const double* t0, double* out,
+                                                  const double* weights, int N, int Nq, int na, int nb,
+                                                  int64_t offb, int64_t wstride, float scale, int last,
+                                                  int gridx, int gridy, int W) {
+  constexpr int BM = 32 * MI, BN = 64 * NI, BNC = BN / 2;
+  constexpr int AV = 2 * BM * 4, AVT = (AV + 511) / 512;     // 16-B vectors per stage: A
+  constexpr int BV = 4 * BNC * 2, BVT = (BV + 511) / 512;    //                         B
+  __shared__ __attribute__((aligned(16))) u16 As[2][BM * TH_LDK];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][BN * TH_LDK];
   const TileId tl = tile_id(gridx, gridy, W, gridy);
   const int w = tl.z;
   if (weights[w] <= 1e-8) return;
-  const int row0 = tl.y * TY_BM, q0 = tl.x * TY_BNC, ncolc = na + nb;
+  const int row0 = tl.y * BM, q0 = tl.x * BNC, ncolc = na + nb;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 2, wn = wave & 3;
-  const double* Pre = P + (int64_t)(2 * w) * N * N;
-  const double* Pim = Pre + (int64_t)N * N;
-  const double* src = tsrc + (int64_t)w * wstride;
-  // A: 128 rows x 16 orbitals of P_re and P_im: thread -> (row, pair of orbitals), 2 per thread
-  // B: 16 orbitals x 128 complex columns of t: thread -> (orbital, column), 4 per thread
-  double2 pre[2], pim[2], tb[4];
-  auto fetch = [&](int k0) {
+  const int nch = Nq >> 4;
+  const int64_t vper = (int64_t)nch * N * 32;     // one walker's V plane
+  const int64_t uplane = (int64_t)ncolc * Nq;     // one u plane
+  const u16* vh = Vh + (int64_t)w * vper;
+  const u16* vl = Vl + (int64_t)w * vper;
+  const u16* ui = Ui + (int64_t)w * 4 * uplane;
+  uint4 ra[AVT], rb[BVT];
+  auto fetch = [&](int ch) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int e = tid + 512 * h, r = e >> 3, kp = (e & 7) * 2, i = row0 + r, k = k0 + kp;
-      pre[h] = double2{0.0, 0.0};
-      pim[h] = double2{0.0, 0.0};
-      if (i < N) {
-        const int64_t o = (int64_t)i * N + k;
-        if (k < N) { pre[h].x = Pre[o]; pim[h].x = Pim[o]; }
-        if (k + 1 < N) { pre[h].y = Pre[o + 1]; pim[h].y = Pim[o + 1]; }
+    for (int h = 0; h < AVT; ++h) {
+      const int e = tid + 512 * h;
+      ra[h] = uint4{0u, 0u, 0u, 0u};
+      if (e < AV) {
+        const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3, i = row0 + r;
+        if (i < N) ra[h] = *reinterpret_cast<const uint4*>((pl ? vl : vh) + ((int64_t)ch * N + i) * 32 + 8 * v);
       }
     }
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const int e = tid + 512 * h, kk = e & (TB_KC - 1), qc = e >> 4;
-      const int kr = k0 + kk, q = q0 + qc;
-      tb[h] = double2{0.0, 0.0};
-      if (kr < N && q < ncolc) {
-        const double* rowp = q < na ? src + 2 * ((int64_t)kr * na + q) : src + 2 * (offb + (int64_t)kr * nb + (q - na));
-        tb[h] = *reinterpret_cast<const double2*>(rowp);
+    for (int h = 0; h < BVT; ++h) {
+      const int e = tid + 512 * h;
+      rb[h] = uint4{0u, 0u, 0u, 0u};
+      if (e < BV) {
+        const int pl = e / (BNC * 2), rem = e - pl * (BNC * 2), q = q0 + (rem >> 1);
+        if (q < ncolc)
+          rb[h] = *reinterpret_cast<const uint4*>(ui + pl * uplane + ((int64_t)ch * ncolc + q) * 16 + 8 * (rem & 1));
       }
     }
-  };
-  auto put = [&](u16* hi, u16* lo, int idx, double x) {
-    u16 a, b;
-    split_bf16(x, a, b);
-    hi[idx] = a;
-    lo[idx] = b;
   };
   auto stage = [&]() {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int e = tid + 512 * h, r = e >> 3, kp = (e & 7) * 2;
-      // [A_re | A_im] = [-P_im | P_re]
-      put(As[0], As[1], r * TB_LDK + kp, -pim[h].x);
-      put(As[0], As[1], r * TB_LDK + kp + 1, -pim[h].y);
-      put(As[0], As[1], r * TB_LDK + TB_KC + kp, pre[h].x);
-      put(As[0], As[1], r * TB_LDK + TB_KC + kp + 1, pre[h].y);
+    for (int h = 0; h < AVT; ++h) {
+      const int e = tid + 512 * h;
+      if (e < AV) {
+        const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3;
+        *reinterpret_cast<uint4*>(As[pl] + r * TH_LDK + 8 * v) = ra[h];
+      }
     }
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const int e = tid + 512 * h, kk = e & (TB_KC - 1), qc = e >> 4;
-      // columns (q, re) and (q, im): [[T_re, T_im], [-T_im, T_re]]
-      put(Bs[0], Bs[1], (2 * qc) * TB_LDK + kk, tb[h].x);
-      put(Bs[0], Bs[1], (2 * qc + 1) * TB_LDK + kk, tb[h].y);
-      put(Bs[0], Bs[1], (2 * qc) * TB_LDK + TB_KC + kk, -tb[h].y);
-      put(Bs[0], Bs[1], (2 * qc + 1) * TB_LDK + TB_KC + kk, tb[h].x);
+    for (int h = 0; h < BVT; ++h) {
+      const int e = tid + 512 * h;
+      if (e < BV) {
+        // plane pl = 2*comp + (lo); columns (q, re) = [u_re | -u_im], (q, im) = [u_im | u_re]
+        const int pl = e / (BNC * 2), rem = e - pl * (BNC * 2), qc = rem >> 1, half = rem & 1;
+        u16* dst = Bs[pl & 1];
+        const uint4 x = rb[h];
+        if (pl < 2) {
+          *reinterpret_cast<uint4*>(dst + (2 * qc) * TH_LDK + 8 * half) = x;
+          *reinterpret_cast<uint4*>(dst + (2 * qc + 1) * TH_LDK + 16 + 8 * half) = x;
+        } else {
+          const uint4 nx = uint4{x.x ^ 0x80008000u, x.y ^ 0x80008000u, x.z ^ 0x80008000u, x.w ^ 0x80008000u};
+          *reinterpret_cast<uint4*>(dst + (2 * qc + 1) * TH_LDK + 8 * half) = x;
+          *reinterpret_cast<uint4*>(dst + (2 * qc) * TH_LDK + 16 + 8 * half) = nx;
+        }
+      }
     }
   };
-  f32x4 c[4][4];
+  f32x4 c[MI][NI];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) c[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int nst = (N + TB_KC - 1) / TB_KC;
+    for (int j = 0; j < NI; ++j) c[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   fetch(0);
-  for (int st = 0; st < nst; ++st) {
+  for (int ch = 0; ch < nch; ++ch) {
     stage();
     __syncthreads();
-    if (st + 1 < nst) fetch((st + 1) * TB_KC);
+    if (ch + 1 < nch) fetch(ch + 1);
     const int koff = 8 * (lane >> 4);
-    bf16x8 a[2][4], b[2][4];
+    bf16x8 b[2][NI];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a[h][i] = *reinterpret_cast<const bf16x8*>(As[h] + (wm * 64 + i * 16 + (lane & 15)) * TB_LDK + koff);
-        b[h][i] = *reinterpret_cast<const bf16x8*>(Bs[h] + (wn * 64 + i * 16 + (lane & 15)) * TB_LDK + koff);
-      }
+      for (int j = 0; j < NI; ++j)
+        b[h][j] = *reinterpret_cast<const bf16x8*>(Bs[h] + (wn * 16 * NI + j * 16 + (lane & 15)) * TH_LDK + koff);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < MI; ++i) {
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As[0] + (wm * 16 * MI + i * 16 + (lane & 15)) * TH_LDK + koff);
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As[1] + (wm * 16 * MI + i * 16 + (lane & 15)) * TH_LDK + koff);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], c[i][j], 0, 0, 0);
-        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], c[i][j], 0, 0, 0);
-        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], c[i][j], 0, 0, 0);
+      for (int j = 0; j < NI; ++j) {
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[0][j], c[i][j], 0, 0, 0);
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[1][j], c[i][j], 0, 0, 0);
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b[0][j], c[i][j], 0, 0, 0);
       }
+    }
     __syncthreads();
   }
-  double* dst = tdst + (int64_t)w * wstride;
-  double* ac = acc + (int64_t)w * wstride;
+  // lane holds rows 4*(lane/16) + r (r = 0..3) of column lane % 16: four consecutive
+  // orbitals of one chunk
+  const u16* t0w = T0b + (int64_t)w * 4 * uplane;
+  u16* uo = Uo + (int64_t)w * 4 * uplane;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < MI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NI; ++j) {
+      const int rowb = row0 + wm * 16 * MI + i * 16 + 4 * (lane >> 4);
+      const int cr = wn * 16 * NI + j * 16 + (lane & 15);
+      const int q = q0 + (cr >> 1), comp = cr & 1;
+      if (rowb >= N || q >= ncolc) continue;
+      if (!last) {
+        const int64_t o = 2 * comp * uplane + ((int64_t)(rowb >> 4) * ncolc + q) * 16 + (rowb & 15);
+        const uint2 th = *reinterpret_cast<const uint2*>(t0w + o);
+        const uint2 tlo = *reinterpret_cast<const uint2*>(t0w + o + uplane);
+        const unsigned hw[4] = {th.x & 0xffffu, th.x >> 16, th.y & 0xffffu, th.y >> 16};
+        const unsigned lw[4] = {tlo.x & 0xffffu, tlo.x >> 16, tlo.y & 0xffffu, tlo.y >> 16};
+        unsigned oh[4], ol[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = row0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;  // bf16 D layout
-        const int cr = wn * 64 + j * 16 + (lane & 15);
-        const int q = q0 + (cr >> 1), comp = cr & 1;
-        if (row < N && q < ncolc) {
+        for (int r = 0; r < 4; ++r) {
+          float x = bf_bits2f((u16)hw[r]) + bf_bits2f((u16)lw[r]) + c[i][j][r] * scale;
+          if (rowb + r >= N) x = 0.f;   // orbital pads stay zero
+          const u16 h = f2bf_bits(x);
+          oh[r] = h;
+          ol[r] = f2bf_bits(x - bf_bits2f(h));
+        }
+        *reinterpret_cast<uint2*>(uo + o) = uint2{oh[0] | (oh[1] << 16), oh[2] | (oh[3] << 16)};
+        *reinterpret_cast<uint2*>(uo + o + uplane) = uint2{ol[0] | (ol[1] << 16), ol[2] | (ol[3] << 16)};
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rowb + r;
+          if (row >= N) break;
           const int64_t a = (q < na ? 2 * ((int64_t)row * na + q) : 2 * (offb + (int64_t)row * nb + (q - na))) + comp;
-          const double x = (double)c[i][j][r] * scale;
-          dst[a] = x;
-          ac[a] += x;
+          out[(int64_t)w * wstride + a] = t0[(int64_t)w * wstride + a] + (double)c[i][j][r] * (double)scale;
         }
       }
+    }
+}
+
+// V P -> the split [-P_im | P_re] chunks of k_taylor_h (used when the VHS ran in FP64)
+__global__ void k_v_split(const double* P, u16* Vh, u16* Vl, const double* weights, int N, int Nq, int W) {
+  const int nch = Nq >> 4;
+  const int64_t per = (int64_t)nch * N * 32, total = per * W;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int w = (int)(e / per);
+    if (weights[w] <= 1e-8) continue;
+    const int64_t r = e - (int64_t)w * per;
+    const int ch = (int)(r / ((int64_t)N * 32));
+    const int rem = (int)(r - (int64_t)ch * N * 32), i = rem >> 5, pos = rem & 31;
+    const int k = ch * 16 + (pos & 15);
+    double x = 0.0;
+    if (k < N) {
+      const double* Pw = P + (int64_t)(2 * w) * N * N;
+      x = pos < 16 ? -Pw[(int64_t)N * N + (int64_t)i * N + k] : Pw[(int64_t)i * N + k];
+    }
+    u16 h, l;
+    split_bf16(x, h, l);
+    Vh[e] = h;
+    Vl[e] = l;
+  }
+}
+void launch_v_split(const DevState& s, hipStream_t st) {
+  const int64_t total = (int64_t)(s.Nq >> 4) * s.N * 32 * s.W;
+  hipLaunchKernelGGL(k_v_split, dim3((unsigned)std::min<int64_t>(cdivu(total, 256), 65535)), dim3(256), 0, st, s.P,
+                     s.Vh, s.Vl, s.weights, s.N, s.Nq, s.W);
 }
 
 // psi <- E (sum_{k<=5} V^k / k!) E psi  for every live walker
 void launch_expmul_pre(const DevState& s, hipStream_t st) {
   const int64_t ws = 2 * s.ms;
+  const bool split = s.prec_prop != AFQ_PREC_FP64;
+  const int ncolc = s.det_n[0] + (s.ndets_prop > 1 ? s.det_n[1] : 0);
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
     const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
     hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.psi, s.t0, s.accb, s.weights, s.N,
-                       n, s.W, ws, 2 * s.det_off[d], 0, gx, gy);
+                       n, s.W, ws, 2 * s.det_off[d], split ? 2 : 0, gx, gy, s.Ub[2], d ? s.det_n[0] : 0, ncolc,
+                       s.Nq);
   }
+}
+template <int MI, int NI>
+static void taylor_split_steps(const DevState& s, hipStream_t st) {
+  const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
+  const int gx = (int)cdivu(na + nb, 32 * NI), gy = (int)cdivu(s.N, 32 * MI);
+  const u16* src = s.Ub[2];
+  for (int p = 5; p >= 1; --p) {
+    u16* dst = s.Ub[p & 1];
+    hipLaunchKernelGGL((k_taylor_h<MI, NI>), dim3(gx * gy * s.W), dim3(512), 0, st, s.Vh, s.Vl, src, dst, s.Ub[2],
+                       s.t0, s.accb, s.weights, s.N, s.Nq, na, nb, s.det_off[1], 2 * s.ms,
+                       (float)(sqrt(s.dt) / p), p == 1 ? 1 : 0, gx, gy, s.W);
+    src = dst;
+  }
+}
+// 128- or 160-row tiles (variant 1 / 2, or the one that pads N less), and 32, 64 or 128
+// complex columns per tile
+template <int MI>
+static void taylor_split_cols(const DevState& s, hipStream_t st) {
+  const int ncolc = s.det_n[0] + (s.ndets_prop > 1 ? s.det_n[1] : 0);
+  if (ncolc <= 32) taylor_split_steps<MI, 1>(s, st);
+  else if (ncolc <= 64) taylor_split_steps<MI, 2>(s, st);
+  else taylor_split_steps<MI, 4>(s, st);
+}
+static void launch_taylor_split(const DevState& s, hipStream_t st) {
+  if (s.prec_vhs == AFQ_PREC_FP64) launch_v_split(s, st);
+  int v = s.taylor_variant;
+  if (v == 0) {
+    const int64_t p4 = (int64_t)cdivu(s.N, 128) * 128, p5 = (int64_t)cdivu(s.N, 160) * 160;
+    v = p5 < p4 ? 2 : 1;
+  }
+  if (v == 2) taylor_split_cols<5>(s, st);
+  else taylor_split_cols<4>(s, st);
 }
 void launch_taylor(const DevState& s, hipStream_t st) {
   const int64_t ws = 2 * s.ms;
@@ -535,12 +643,12 @@ void launch_taylor(const DevState& s, hipStream_t st) {
   double* dst = s.t1;
   const bool wide = s.taylor_variant == 0 ? n2a + n2b > 128 : s.taylor_variant == 2;
   const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
+  if (s.prec_prop != AFQ_PREC_FP64) {
+    launch_taylor_split(s, st);
+    return;
+  }
   for (int p = 1; p <= 5; ++p) {
-    if (s.prec_prop != AFQ_PREC_FP64) {
-      const int gx = (int)cdivu(na + nb, TY_BNC), gy = (int)cdivu(s.N, TY_BM);
-      hipLaunchKernelGGL(k_taylor_bf16, dim3(gx * gy * s.W), dim3(512), 0, st, s.P, src, dst, s.accb, s.weights,
-                         s.N, na, nb, s.det_off[1], ws, sq / p, gx, gy, s.W);
-    } else if (wide) {
+    if (wide) {
       const int gx = (int)cdivu(na + nb, TY_BNC), gy = (int)cdivu(s.N, TY_BM);
       if (s.N % 2 == 0)
         hipLaunchKernelGGL(k_taylor<true>, dim3(gx * gy * s.W), dim3(512), 0, st, s.P, src, dst, s.accb, s.weights,
@@ -562,7 +670,7 @@ void launch_expmul_post(const DevState& s, hipStream_t st) {
     const int n = s.det_n[d];
     const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
     hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.accb, s.psi, nullptr, s.weights,
-                       s.N, n, s.W, ws, 2 * s.det_off[d], 1, gx, gy);
+                       s.N, n, s.W, ws, 2 * s.det_off[d], 1, gx, gy, nullptr, 0, 0, 0);
   }
 }
 void launch_propagate(const DevState& s, hipStream_t st) {
@@ -1479,11 +1587,13 @@ __global__ void k_fields_split(const double* fields, u16* hi, u16* lo, int W, in
   }
 }
 
-// P[w][c] (both triangles) = sum_g field[w][g][c] L_g from the split operands
+// P[w][c] (both triangles) = sum_g field[w][g][c] L_g from the split operands.  With
+// Vh != nullptr the result goes straight to the split [-P_im | P_re] chunks that the
+// split-BF16 Taylor series reads (Vh/Vl[w][chunk][i][32]) instead of the FP64 P.
 template <int NS>
 __global__ __launch_bounds__(256, 2) void k_vhs_bf16(const u16* Fh, const u16* Fl, const u16* Lh, const u16* Ll,
                                                      double* P, const double* weights, int W, int64_t T, int N,
-                                                     int Gp, int gridx, int gridy) {
+                                                     int Gp, int gridx, int gridy, u16* Vh, u16* Vl, int Nq) {
   using Eng = MmaBF16<128, 128, 2, 2, NS, NS>;
   __shared__ __attribute__((aligned(16))) unsigned char lds[Eng::LDS_BYTES];
   const TileId tl = tile_id(gridx, gridy, 1, 8);
@@ -1511,6 +1621,18 @@ __global__ __launch_bounds__(256, 2) void k_vhs_bf16(const u16* Fh, const u16* F
     if ((int64_t)(i + 1) * (i + 2) / 2 <= t) ++i;
     if ((int64_t)i * (i + 1) / 2 > t) --i;
     const int j = (int)(t - (int64_t)i * (i + 1) / 2);
+    if (Vh) {
+      const int w = q >> 1, comp = q & 1;
+      const float x = comp ? -v : v;
+      const u16 h = f2bf_bits(x), l = f2bf_bits(x - bf_bits2f(h));
+      const int64_t base = (int64_t)w * (Nq >> 4) * N * 32;
+      const int off = comp ? 0 : 16;
+      const int64_t e1 = base + ((int64_t)(j >> 4) * N + i) * 32 + off + (j & 15);
+      const int64_t e2 = base + ((int64_t)(i >> 4) * N + j) * 32 + off + (i & 15);
+      Vh[e1] = h; Vl[e1] = l;
+      Vh[e2] = h; Vl[e2] = l;
+      return;
+    }
     double* Pq = P + (int64_t)q * N * N;
     Pq[(int64_t)i * N + j] = (double)v;
     Pq[(int64_t)j * N + i] = (double)v;
@@ -1522,12 +1644,14 @@ void launch_vhs_lp(const DevState& s, int prec, hipStream_t st) {
   hipLaunchKernelGGL(k_fields_split, dim3((unsigned)std::min<int64_t>(cdivu(tot, 256), 65535)), dim3(256), 0, st,
                      s.fields, s.Fb_hi, s.Fb_lo, s.W, s.G, s.Gp);
   const int gx = (int)cdivu(s.T, 128), gy = (int)cdivu(2 * (int64_t)s.W, 128);
+  u16* vh = s.prec_prop != AFQ_PREC_FP64 ? s.Vh : nullptr;
+  u16* vl = s.prec_prop != AFQ_PREC_FP64 ? s.Vl : nullptr;
   if (prec == AFQ_PREC_BF16X3)
     hipLaunchKernelGGL(k_vhs_bf16<2>, dim3(gx * gy), dim3(256), 0, st, s.Fb_hi, s.Fb_lo, s.Lt_hi, s.Lt_lo, s.P,
-                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy);
+                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy, vh, vl, s.Nq);
   else
     hipLaunchKernelGGL(k_vhs_bf16<1>, dim3(gx * gy), dim3(256), 0, st, s.Fb_hi, s.Fb_lo, s.Lt_hi, s.Lt_lo, s.P,
-                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy);
+                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy, vh, vl, s.Nq);
 }
 
 // E_L = ecore + sum_s m_s e1_s + 1/2 sum_{g<GE} fb_g^2 - 1/2 sum_s m_s X_s
