@@ -174,8 +174,14 @@ def stage_rooflines(spec, path, n_exact):
         add("vhs", "k_fields_split + k_vhs_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * T * G,
             4.0 * G * T + W * (24.0 * G + 8.0 * ((N + 15) // 16 * 16) * N), 1, "bf16",
             symbol="void k_vhs_bf16<2>", passes=3)
-    add("onebody", "k_expmul (E psi, FP64 MFMA)", W * 8.0 * N * N * ncol, W * 3 * 16.0 * ms, 2 * len(nsp), "fp64",
-        symbol="k_expmul")
+    if path.get("fb_vhs_prec", "fp64") == "fp64":
+        add("onebody", "k_expmul (E psi, FP64 MFMA)", W * 8.0 * N * N * ncol, W * 3 * 16.0 * ms, 2 * len(nsp),
+            "fp64", symbol="k_expmul")
+    else:
+        # E x = x + (E - I) x: x read (FP64), E x written (FP64; before the series also split)
+        mi = 5 if -(-N // 160) * 160 < -(-N // 128) * 128 else 4
+        add("onebody", "k_onebody_split (E x = x + D x, D x in 3-pass split BF16 MFMA)", W * 8.0 * N * N * ncol,
+            W * (40.0 + 32.0) * ms, 2 * len(nsp), "bf16", symbol="void k_onebody_split<%d>" % mi, passes=3)
     add("green", "k_green (FP64 MFMA + Gauss-Jordan)",
         W * sum(4 * n * n * N + 16 * n ** 3 + 8 * N * n * n for n in nsp), W * 2 * 16.0 * ms, 1, "fp64",
         symbol="k_green")

@@ -180,7 +180,7 @@ void free_all(DevState& s) {
   if (s.idx) (void)hipFree(s.idx);
   if (s.ncap) (void)hipFree(s.ncap);
   unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo, s.Lt_hi, s.Lt_lo, s.Fb_hi, s.Fb_lo,
-                          s.Vh, s.Vl, s.Ub[0], s.Ub[1], s.Ub[2]};
+                          s.Vh, s.Vl, s.Ub[0], s.Ub[1], s.Ub[2], s.Dh, s.Dl};
   for (unsigned short* p : hp)
     if (p) (void)hipFree(p);
 }
@@ -224,6 +224,10 @@ void ensure_prop_lowprec(DevState& s, hipStream_t st) {
   afq::launch_fill(reinterpret_cast<double*>(s.Vh), nv / 4, 0.0, st);
   afq::launch_fill(reinterpret_cast<double*>(s.Vl), nv / 4, 0.0, st);
   for (int k = 0; k < 3; ++k) afq::launch_fill(reinterpret_cast<double*>(s.Ub[k]), nu / 4, 0.0, st);
+  const int64_t nd = (int64_t)s.N * afq::onebody_pitch(s.N);
+  s.Dh = dalloc<unsigned short>(nd);
+  s.Dl = dalloc<unsigned short>(nd);
+  afq::launch_d_split(s, st);   // again at every afq_prep_prop
   launch_check("prop split buffers");
 }
 
@@ -246,7 +250,7 @@ void free_temporary(const DevState& t) {
   z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
   z.Rb_hi = z.Rb_lo = z.Tb_hi = z.Tb_lo = nullptr;
   z.Lt_hi = z.Lt_lo = z.Fb_hi = z.Fb_lo = nullptr;
-  z.Vh = z.Vl = z.Ub[0] = z.Ub[1] = z.Ub[2] = nullptr;
+  z.Vh = z.Vl = z.Ub[0] = z.Ub[1] = z.Ub[2] = z.Dh = z.Dl = nullptr;
   free_all(z);
 }
 
@@ -651,6 +655,7 @@ int afq_prep_prop(afq_ctx* c, const double* ref_up, const double* ref_dn, const 
   s.prop_const_im = 0.0;
   upload(s.mf, mf.data(), mf.size(), c->stream);
   upload(s.E, E.data(), E.size(), c->stream);
+  if (s.Dh) afq::launch_d_split(s, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "prep sync");
   c->prepped = true;
   return AFQ_OK;

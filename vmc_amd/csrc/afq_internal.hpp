@@ -88,9 +88,12 @@ struct DevState {
   //   Vh/Vl  [w][chunk][i][32]: hi/lo of [-P_im | P_re] over the chunk's 16 orbitals
   //   Ub[0], Ub[1] (Horner terms) and Ub[2] (E psi): [w][plane][chunk][q][16], planes
   //          re_hi, re_lo, im_hi, im_lo; q runs over the na + nb propagated columns
+  //   Dh/Dl  D = E - I (the one-body factor less the identity), N rows of pitch
+  //          onebody_pitch(N)
   int Nq = 0;
   unsigned short* Vh = nullptr; unsigned short* Vl = nullptr;
   unsigned short* Ub[3] = {nullptr, nullptr, nullptr};
+  unsigned short* Dh = nullptr; unsigned short* Dl = nullptr;
   // replayed (hipGraph) steps: step counter, energy shift and comb uniform live in
   // device memory, and the comb copies back instead of swapping buffers
   const unsigned long long* d_step = nullptr;
@@ -153,7 +156,9 @@ void launch_qr(const DevState& s, hipStream_t st);
 void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st);
 void launch_energy(const DevState& s, const int* sel, int nsel, double delta, hipStream_t st);
 void launch_comb(DevState& s, double zeta, hipStream_t st);
+inline int onebody_pitch(int N) { return (N + 31) / 32 * 32; }
 void launch_v_split(const DevState& s, hipStream_t st);
+void launch_d_split(const DevState& s, hipStream_t st);
 void launch_copy_det0_to_det1(const DevState& s, double* dst, const double* src, hipStream_t st);
 
 void launch_pack_chol(const double* L, double* Lp, int N, int G, int64_t T, bool round_f32, hipStream_t st);

@@ -203,13 +203,11 @@ struct DetColsB {  // B(k, q) = src[w][det d][k][cc], q = w*2n + cc
   }
 };
 constexpr int EX_BM = 64, EX_BN = 128;
-// mode 0: dst1 = dst2 = E src (start of the Taylor series); mode 1: dst1 = E src for live walkers;
-// mode 2: dst1 = E src and its split-BF16 copy in t0b (columns from coloff of the ncolc
-// propagated columns, layout [w][re_hi, re_lo, im_hi, im_lo][orbital chunk of 16][q][16])
+// mode 0: dst1 = dst2 = E src (start of the Taylor series); mode 1: dst1 = E src for live walkers
 __global__ __launch_bounds__(256) void k_expmul(const double* E, const double* src, double* dst1,
                                                 double* dst2, const double* weights, int N, int n,
                                                 int W, int64_t wstride, int64_t doff, int mode, int gridx,
-                                                int gridy, u16* t0b, int coloff, int ncolc, int Nq) {
+                                                int gridy) {
   using Eng = MmaF64<EX_BM, EX_BN, 2, 2, false, true>;
   __shared__ double lds[Eng::LDS_DOUBLES];
   const TileId tl = tile_id(gridx, gridy, 1, gridy);
@@ -225,16 +223,6 @@ __global__ __launch_bounds__(256) void k_expmul(const double* E, const double* s
     const int64_t a = (int64_t)w * wstride + doff + (int64_t)i * n2 + cc;
     dst1[a] = v;
     if (mode == 0) dst2[a] = v;
-    if (mode == 2) {
-      const int64_t plane = (int64_t)ncolc * Nq;
-      const int comp = cc & 1;
-      const int64_t b = (int64_t)w * 4 * plane + 2 * comp * plane + (int64_t)(i >> 4) * ncolc * 16 +
-                        (int64_t)(coloff + (cc >> 1)) * 16 + (i & 15);
-      u16 h, l;
-      split_bf16(v, h, l);
-      t0b[b] = h;
-      t0b[b + plane] = l;
-    }
   });
 }
 
@@ -588,17 +576,197 @@ void launch_v_split(const DevState& s, hipStream_t st) {
                      s.Vh, s.Vl, s.weights, s.N, s.Nq, s.W);
 }
 
+// One-body factor in split BF16 with the identity kept exact: E = I + D with D = E - I
+// small (|D| ~ dt/2 |h1'|), so E x = x + D x takes x in FP64 and only D x through three
+// MFMA passes (hi*hi + hi*lo + lo*hi, FP32 accumulation).  D is split once per context
+// (Dh/Dl, N rows of pitch Nd = N padded to 32, zero pads); x is split while staging.  Columns are the
+// W * 2n real columns of one determinant of every walker (D is real).  mode 0:
+// dst = E x and its split copy in t0b (the first Taylor term, columns from coloff of the
+// ncolc propagated columns); mode 1: dst = E x.  Tile 32*MI rows x 256 columns.
+template <int MI>
+__global__ __launch_bounds__(512) void k_onebody_split(const u16* Dh, const u16* Dl, const double* src, double* dst,
+                                                       const double* weights, int N, int Nq, int Nd, int n, int W,
+                                                       int64_t wstride, int64_t doff, int mode, u16* t0b, int coloff,
+                                                       int ncolc, int gridx, int gridy) {
+  constexpr int BM = 32 * MI, BN = 256;
+  constexpr int AV = 2 * BM * 4, AVT = (AV + 511) / 512;
+  __shared__ __attribute__((aligned(16))) u16 As[2][BM * TH_LDK];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][BN * TH_LDK];
+  const TileId tl = tile_id(gridx, gridy, 1, gridy);
+  const int row0 = tl.y * BM, c0 = tl.x * BN;
+  const int n2 = 2 * n, ncol = W * n2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 2, wn = wave & 3;
+  // B: thread -> column c0 + (tid & 255), orbital groups of 8: (tid >> 8) and 2 + (tid >> 8)
+  const int bc = c0 + (tid & 255);
+  const double* bcol = nullptr;
+  if (bc < ncol) {
+    const int w = bc / n2;
+    bcol = src + (int64_t)w * wstride + doff + (bc - w * n2);
+  }
+  uint4 ra[AVT];
+  double rb[2][8];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < AVT; ++h) {
+      const int e = tid + 512 * h;
+      ra[h] = uint4{0u, 0u, 0u, 0u};
+      if (e < AV) {
+        const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3, i = row0 + r;
+        if (i < N) ra[h] = *reinterpret_cast<const uint4*>((pl ? Dl : Dh) + (int64_t)i * Nd + k0 + 8 * v);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int kb = k0 + 8 * ((tid >> 8) + 2 * g);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) rb[g][t] = (bcol && kb + t < N) ? bcol[(int64_t)(kb + t) * n2] : 0.0;
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int h = 0; h < AVT; ++h) {
+      const int e = tid + 512 * h;
+      if (e < AV) {
+        const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3;
+        *reinterpret_cast<uint4*>(As[pl] + r * TH_LDK + 8 * v) = ra[h];
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      unsigned hw[4], lw[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        u16 h0, l0, h1, l1;
+        split_bf16(rb[g][2 * t], h0, l0);
+        split_bf16(rb[g][2 * t + 1], h1, l1);
+        hw[t] = (unsigned)h0 | ((unsigned)h1 << 16);
+        lw[t] = (unsigned)l0 | ((unsigned)l1 << 16);
+      }
+      const int off = (tid & 255) * TH_LDK + 8 * ((tid >> 8) + 2 * g);
+      *reinterpret_cast<uint4*>(Bs[0] + off) = uint4{hw[0], hw[1], hw[2], hw[3]};
+      *reinterpret_cast<uint4*>(Bs[1] + off) = uint4{lw[0], lw[1], lw[2], lw[3]};
+    }
+  };
+  f32x4 c[MI][4];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nst = Nd >> 5;
+  fetch(0);
+  for (int st = 0; st < nst; ++st) {
+    stage();
+    __syncthreads();
+    if (st + 1 < nst) fetch((st + 1) * 32);
+    const int koff = 8 * (lane >> 4);
+    bf16x8 b[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        b[h][j] = *reinterpret_cast<const bf16x8*>(Bs[h] + (wn * 64 + j * 16 + (lane & 15)) * TH_LDK + koff);
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As[0] + (wm * 16 * MI + i * 16 + (lane & 15)) * TH_LDK + koff);
+      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As[1] + (wm * 16 * MI + i * 16 + (lane & 15)) * TH_LDK + koff);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[0][j], c[i][j], 0, 0, 0);
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[1][j], c[i][j], 0, 0, 0);
+        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b[0][j], c[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t plane = (int64_t)ncolc * Nq;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = c0 + wn * 64 + j * 16 + (lane & 15);
+    if (col >= ncol) continue;
+    const int w = col / n2, cc = col - w * n2;
+    if (weights[w] <= 1e-8) continue;
+    const int64_t base = (int64_t)w * wstride + doff + cc;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      const int rowb = row0 + wm * 16 * MI + i * 16 + 4 * (lane >> 4);
+      if (rowb >= N) continue;
+      float y[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rowb + r;
+        y[r] = 0.f;
+        if (row < N) {
+          const double v = src[base + (int64_t)row * n2] + (double)c[i][j][r];
+          dst[base + (int64_t)row * n2] = v;
+          y[r] = (float)v;
+        }
+      }
+      if (mode == 0) {
+        const int comp = cc & 1;
+        const int64_t o = (int64_t)w * 4 * plane + 2 * comp * plane + ((int64_t)(rowb >> 4) * ncolc + coloff + (cc >> 1)) * 16 +
+                          (rowb & 15);
+        unsigned oh[4], ol[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const u16 h = f2bf_bits(y[r]);
+          oh[r] = h;
+          ol[r] = f2bf_bits(y[r] - bf_bits2f(h));
+        }
+        *reinterpret_cast<uint2*>(t0b + o) = uint2{oh[0] | (oh[1] << 16), oh[2] | (oh[3] << 16)};
+        *reinterpret_cast<uint2*>(t0b + o + plane) = uint2{ol[0] | (ol[1] << 16), ol[2] | (ol[3] << 16)};
+      }
+    }
+  }
+}
+
+// D = E - I, split, N rows of pitch Nd (zero pads)
+__global__ void k_d_split(const double* E, u16* Dh, u16* Dl, int N, int Nd) {
+  const int64_t total = (int64_t)N * Nd;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / Nd), k = (int)(e - (int64_t)i * Nd);
+    const double x = k < N ? E[(int64_t)i * N + k] - (i == k ? 1.0 : 0.0) : 0.0;
+    u16 h, l;
+    split_bf16(x, h, l);
+    Dh[e] = h;
+    Dl[e] = l;
+  }
+}
+void launch_d_split(const DevState& s, hipStream_t st) {
+  const int Nd = onebody_pitch(s.N);
+  const int64_t total = (int64_t)s.N * Nd;
+  hipLaunchKernelGGL(k_d_split, dim3((unsigned)std::min<int64_t>(cdivu(total, 256), 4096)), dim3(256), 0, st, s.E,
+                     s.Dh, s.Dl, s.N, Nd);
+}
+template <int MI>
+static void onebody_split(const DevState& s, const double* src, double* dst, int mode, hipStream_t st) {
+  const int64_t ws = 2 * s.ms;
+  const int ncolc = s.det_n[0] + (s.ndets_prop > 1 ? s.det_n[1] : 0);
+  for (int d = 0; d < s.ndets_prop; ++d) {
+    const int n = s.det_n[d];
+    const int gx = (int)cdivu((int64_t)s.W * 2 * n, 256), gy = (int)cdivu(s.N, 32 * MI);
+    hipLaunchKernelGGL((k_onebody_split<MI>), dim3(gx * gy), dim3(512), 0, st, s.Dh, s.Dl, src, dst, s.weights, s.N,
+                       s.Nq, onebody_pitch(s.N), n, s.W, ws, 2 * s.det_off[d], mode, s.Ub[2], d ? s.det_n[0] : 0,
+                       ncolc, gx, gy);
+  }
+}
+static void launch_onebody_split(const DevState& s, const double* src, double* dst, int mode, hipStream_t st) {
+  const int64_t p4 = (int64_t)cdivu(s.N, 128) * 128, p5 = (int64_t)cdivu(s.N, 160) * 160;
+  if (p5 < p4) onebody_split<5>(s, src, dst, mode, st);
+  else onebody_split<4>(s, src, dst, mode, st);
+}
+
 // psi <- E (sum_{k<=5} V^k / k!) E psi  for every live walker
 void launch_expmul_pre(const DevState& s, hipStream_t st) {
+  if (s.prec_prop != AFQ_PREC_FP64) {
+    launch_onebody_split(s, s.psi, s.t0, 0, st);
+    return;
+  }
   const int64_t ws = 2 * s.ms;
-  const bool split = s.prec_prop != AFQ_PREC_FP64;
-  const int ncolc = s.det_n[0] + (s.ndets_prop > 1 ? s.det_n[1] : 0);
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
     const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
     hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.psi, s.t0, s.accb, s.weights, s.N,
-                       n, s.W, ws, 2 * s.det_off[d], split ? 2 : 0, gx, gy, s.Ub[2], d ? s.det_n[0] : 0, ncolc,
-                       s.Nq);
+                       n, s.W, ws, 2 * s.det_off[d], 0, gx, gy);
   }
 }
 template <int MI, int NI>
@@ -665,12 +833,16 @@ void launch_taylor(const DevState& s, hipStream_t st) {
   }
 }
 void launch_expmul_post(const DevState& s, hipStream_t st) {
+  if (s.prec_prop != AFQ_PREC_FP64) {
+    launch_onebody_split(s, s.accb, s.psi, 1, st);
+    return;
+  }
   const int64_t ws = 2 * s.ms;
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
     const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
     hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.accb, s.psi, nullptr, s.weights,
-                       s.N, n, s.W, ws, 2 * s.det_off[d], 1, gx, gy, nullptr, 0, 0, 0);
+                       s.N, n, s.W, ws, 2 * s.det_off[d], 1, gx, gy);
   }
 }
 void launch_propagate(const DevState& s, hipStream_t st) {
