@@ -174,7 +174,7 @@ struct Stage {
 void free_all(DevState& s) {
   double* ptrs[] = {s.E, s.Lp, s.R, s.phi[0], s.phi[1], s.hphi[0], s.hphi[1], s.mf, s.psi, s.theta, s.fb,
                     s.logO, s.e1, s.logdet, s.weights, s.factors, s.psi2, s.theta2, s.fb2, s.logO2, s.e12,
-                    s.fields, s.sft, s.P, s.t0, s.t1, s.accb, s.draws, s.Xp, s.eloc, s.ho, s.cum};
+                    s.fields, s.sft, s.P, s.t0, s.t1, s.accb, s.draws, s.Xp, s.eloc, s.ho, s.cum, s.Sg};
   for (double* p : ptrs)
     if (p) (void)hipFree(p);
   if (s.idx) (void)hipFree(s.idx);
@@ -278,6 +278,7 @@ void alloc_walkers(DevState& s) {
   s.accb = dalloc<double>(2 * W * s.ms);
   s.draws = dalloc<double>(W * s.G);
   s.Xp = dalloc<double>(2 * (int64_t)s.nslots * W * s.ngt_cap);
+  s.Sg = dalloc<double>((int64_t)W * s.nslots * 64 * 128);
   s.eloc = dalloc<double>(W);
   s.ho = dalloc<double>(4 * W);
   s.cum = dalloc<double>(W);

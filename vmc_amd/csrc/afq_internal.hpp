@@ -67,6 +67,7 @@ struct DevState {
   double* P = nullptr;       // W x 2 x N x N (VHS: sum_g Re/Im(field) L_g, both triangles)
   double* t0 = nullptr; double* t1 = nullptr; double* accb = nullptr;  // W x ms complex
   double* draws = nullptr;   // W x G (replay)
+  double* Sg = nullptr;      // W x nslots x 64 x 128: overlap matrices S = Phi^T psi, then S^-1
   double* Xp = nullptr;      // nslots x W x ngt complex exchange partials
   int ngt_cap = 0;
   double* eloc = nullptr;    // W

@@ -853,97 +853,151 @@ void launch_propagate(const DevState& s, hipStream_t st) {
 
 // ------------------------------------------------------------------ Green's function
 constexpr int GN_MAX = 64;
-struct GreenArgs {
-  const double* psi; double* theta; double* logdet; double* e1; const double* weights;
-  const double* phi0; const double* phi1; const double* hphi0; const double* hphi1;
-  int64_t ms, Ktot; int64_t det_off0, det_off1; int N, nslots; Slot s0, s1; int all;
-};
 
-__global__ __launch_bounds__(256) void k_green(GreenArgs a) {
+// The Green's function in three batched kernels:
+//  1. k_overlap: S = Phi_s^T psi for every walker of a slot as one GEMM over the
+//     W * 2n real columns (FP64 MFMA), into Sg[w][slot] (64 x 128 doubles each);
+//  2. k_invert: S^-1 and log det S per (walker, slot) by Gauss-Jordan elimination with
+//     partial pivoting, the matrix held in registers (lane = row, wave q = complex
+//     columns 16q..16q+15), so a workgroup needs only 1 KB of LDS;
+//  3. k_theta: theta = psi S^-1 (FP64 MFMA, S^-1 read from Sg) and the one-body
+//     partial e1 = sum theta o (h1 Phi).
+struct GreenArgs {
+  const double* psi; double* theta; double* logdet; double* e1; const double* weights; double* Sg;
+  const double* phi0; const double* phi1; const double* hphi0; const double* hphi1;
+  int64_t ms, Ktot; int64_t det_off0, det_off1; int N, nslots, W; Slot s0, s1; int all;
+};
+constexpr int SG_LD = 2 * GN_MAX;                       // doubles per row of an Sg matrix
+constexpr int64_t SG_MAT = (int64_t)GN_MAX * SG_LD;     // doubles per (walker, slot)
+
+__global__ __launch_bounds__(256, 3) void k_overlap(GreenArgs a, int si, int gridx) {
   using ES = MmaF64<64, 128, 2, 2, true, true>;
-  using ET = MmaF64<64, 128, 2, 2, false, true>;
-  constexpr int STAGE = ES::LDS_DOUBLES > ET::LDS_DOUBLES ? ES::LDS_DOUBLES : ET::LDS_DOUBLES;
-  __shared__ double stage[STAGE];
-  __shared__ cplx Sm[GN_MAX * GN_MAX];
-  __shared__ cplx fcol[GN_MAX];
-  __shared__ double red[16];
-  __shared__ int piv[GN_MAX];
-  __shared__ int pidx;
+  __shared__ double stage[ES::LDS_DOUBLES];
+  const Slot sl = si == 0 ? a.s0 : a.s1;
+  const int n = sl.n, n2 = 2 * n, ncol = a.W * n2;
+  const int q0 = blockIdx.x * 128;
+  const double* phi = sl.phi == 0 ? a.phi0 : a.phi1;
+  const int64_t doff = 2 * (sl.det == 0 ? a.det_off0 : a.det_off1);
+  ES e;
+  e.run([&](int r, int k) { return (r < n && k < a.N) ? phi[(int64_t)k * n + r] : 0.0; },
+        DetColsB{a.psi, 2 * a.ms, doff, a.N, n2, q0, ncol}, a.N, stage);
+  e.store([&](int r, int c, double v) {
+    const int q = q0 + c;
+    if (r >= n || q >= ncol) return;
+    const int w = q / n2, cc = q - w * n2;
+    if (!a.all && a.weights[w] <= 1e-8) return;
+    a.Sg[((int64_t)w * a.nslots + si) * SG_MAT + (int64_t)r * SG_LD + cc] = v;
+  });
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__global__ __launch_bounds__(256, 3) void k_invert(GreenArgs a) {
+  __shared__ cplx colb[2][GN_MAX];
+  __shared__ cplx pvs[GN_MAX];
+  __shared__ int pk[GN_MAX];
   const int w = blockIdx.x, si = blockIdx.y;
   if (!a.all && a.weights[w] <= 1e-8) return;
   const Slot sl = si == 0 ? a.s0 : a.s1;
-  const int n = sl.n, N = a.N, n2 = 2 * n, tid = threadIdx.x;
-  const double* psi = a.psi + (int64_t)w * 2 * a.ms + 2 * (sl.det == 0 ? a.det_off0 : a.det_off1);
-  const double* phi = sl.phi == 0 ? a.phi0 : a.phi1;
-  const double* hphi = sl.phi == 0 ? a.hphi0 : a.hphi1;
-  double* Sd = reinterpret_cast<double*>(Sm);
-  // S = Phi^T psi  (n x n complex, row stride 2*GN_MAX doubles)
-  {
-    ES e;
-    e.run([&](int r, int k) { return (r < n && k < N) ? phi[(int64_t)k * n + r] : 0.0; },
-          [&](int k, int c) { return (k < N && c < n2) ? psi[(int64_t)k * n2 + c] : 0.0; }, N, stage);
-    e.store([&](int r, int c, double v) {
-      if (r < n && c < n2) Sd[r * 2 * GN_MAX + c] = v;
-    });
+  const int n = sl.n, tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+  double* S = a.Sg + ((int64_t)w * a.nslots + si) * SG_MAT;
+  cplx x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int col = 16 * q + j;
+    x[j] = cplx{0.0, 0.0};
+    if (lane < n && col < n) x[j] = cplx{S[lane * SG_LD + 2 * col], S[lane * SG_LD + 2 * col + 1]};
+  }
+  bool used = lane >= n;
+  int myk = 0;
+  for (int k = 0; k < n; ++k) {
+    const int o = k >> 4, c = k & 15;
+    if (q == o) {
+      cplx sel = x[0];
+#pragma unroll
+      for (int j = 1; j < 16; ++j)
+        if (j == c) sel = x[j];
+      colb[k & 1][lane] = sel;
+    }
+    __syncthreads();
+    const cplx f = colb[k & 1][lane];
+    // pivot: the unused row of largest |f|, ties to the lower row
+    double v = used ? -1.0 : f.re * f.re + f.im * f.im;
+    int idx = lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(v, off, 64);
+      const int oi = __shfl_xor(idx, off, 64);
+      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    const int p = __builtin_amdgcn_readfirstlane(idx);
+    const cplx pv = colb[k & 1][p];
+    const cplx pinv = cdiv(cplx{1.0, 0.0}, pv);
+    if (lane == p) { used = true; myk = k; }
+    if (tid == 0) {
+      pk[k] = p;
+      pvs[k] = pv;
+    }
+    // row p / pv, and every other row minus f_i times it; the stored column k turns
+    // into column p of the inverse (in place)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const bool kcol = q == o && j == c;
+      cplx r = cmul(cplx{readlane_f64(x[j].re, p), readlane_f64(x[j].im, p)}, pinv);
+      if (kcol) r = pinv;
+      if (lane == p) x[j] = r;
+      else x[j] = csub(kcol ? cplx{0.0, 0.0} : x[j], cmul(f, r));
+    }
   }
   __syncthreads();
-  // in-place Gauss-Jordan inversion with partial pivoting; log det from the pivots
-  double ld_re = 0.0, ld_im = 0.0;
-  int nswap = 0;
-  for (int k = 0; k < n; ++k) {
-    if (tid < 64) {
-      int i = k + tid;
-      double v = -1.0;
-      if (i < n) { const cplx x = Sm[i * GN_MAX + k]; v = x.re * x.re + x.im * x.im; }
-      else i = n;
+  // S^-1[k][p_j] = X[p_k][j]: lane p_k holds row myk, the slot of column j holds column p_j
+  if (lane < n) {
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off, 64);
-        const int oi = __shfl_xor(i, off, 64);
-        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    for (int j = 0; j < 16; ++j) {
+      const int col = 16 * q + j;
+      if (col < n) {
+        const int lc = pk[col];
+        S[myk * SG_LD + 2 * lc] = x[j].re;
+        S[myk * SG_LD + 2 * lc + 1] = x[j].im;
       }
-      if (tid == 0) { pidx = i; piv[k] = i; }
     }
-    __syncthreads();
-    const int p = pidx;
-    if (p != k)
-      for (int j = tid; j < n; j += blockDim.x) {
-        const cplx t = Sm[k * GN_MAX + j];
-        Sm[k * GN_MAX + j] = Sm[p * GN_MAX + j];
-        Sm[p * GN_MAX + j] = t;
-      }
-    __syncthreads();
-    const cplx pv = Sm[k * GN_MAX + k];
-    if (tid == 0) {
-      const cplx l = clog(pv);
-      ld_re += l.re; ld_im += l.im;
-      if (p != k) ++nswap;
-    }
-    const cplx pinv = cdiv(cplx{1.0, 0.0}, pv);
-    for (int i = tid; i < n; i += blockDim.x) fcol[i] = Sm[i * GN_MAX + k];
-    __syncthreads();
-    for (int j = tid; j < n; j += blockDim.x)
-      Sm[k * GN_MAX + j] = (j == k) ? pinv : cmul(Sm[k * GN_MAX + j], pinv);
-    __syncthreads();
-    for (int e = tid; e < n * n; e += blockDim.x) {
-      const int i = e / n, j = e - i * n;
-      if (i == k) continue;
-      const cplx base = (j == k) ? cplx{0.0, 0.0} : Sm[i * GN_MAX + j];
-      Sm[i * GN_MAX + j] = csub(base, cmul(fcol[i], Sm[k * GN_MAX + j]));
-    }
-    __syncthreads();
   }
-  for (int k = n - 1; k >= 0; --k) {
-    const int p = piv[k];
-    if (p != k)
-      for (int i = tid; i < n; i += blockDim.x) {
-        const cplx t = Sm[i * GN_MAX + k];
-        Sm[i * GN_MAX + k] = Sm[i * GN_MAX + p];
-        Sm[i * GN_MAX + p] = t;
-      }
-    __syncthreads();
+  if (q == 0) {
+    // det S = sign(k -> p_k) prod pivots
+    cplx l = lane < n ? clog(pvs[lane]) : cplx{0.0, 0.0};
+    const double ld_re = wave_sum(l.re);
+    double ld_im = wave_sum(l.im);
+    unsigned long long seen = 0ull;
+    int cycles = 0;
+    for (int k = 0; k < n; ++k) {
+      if (seen >> k & 1ull) continue;
+      ++cycles;
+      for (int t = k; t >= 0 && t < n && !(seen >> t & 1ull); t = pk[t]) seen |= 1ull << t;
+    }
+    if ((n - cycles) & 1) ld_im += M_PI;
+    if (lane == 0) {
+      a.logdet[2 * ((int64_t)w * a.nslots + si)] = ld_re;
+      a.logdet[2 * ((int64_t)w * a.nslots + si) + 1] = ld_im;
+    }
   }
-  // theta = psi S^-1 (N x n complex), one-body e1 = sum theta o (h1 Phi)
+}
+
+__global__ __launch_bounds__(256, 2) void k_theta(GreenArgs a) {
+  using ET = MmaF64<64, 128, 2, 2, false, true>;
+  __shared__ double stage[ET::LDS_DOUBLES];
+  __shared__ double red[16];
+  const int w = blockIdx.x, si = blockIdx.y;
+  if (!a.all && a.weights[w] <= 1e-8) return;
+  const Slot sl = si == 0 ? a.s0 : a.s1;
+  const int n = sl.n, N = a.N, n2 = 2 * n;
+  const double* psi = a.psi + (int64_t)w * 2 * a.ms + 2 * (sl.det == 0 ? a.det_off0 : a.det_off1);
+  const double* hphi = sl.phi == 0 ? a.hphi0 : a.hphi1;
+  const double* Si = a.Sg + ((int64_t)w * a.nslots + si) * SG_MAT;
   double e1re = 0.0, e1im = 0.0;
   double* th = a.theta + (int64_t)w * 2 * a.Ktot + 2 * sl.koff;
   for (int r0 = 0; r0 < N; r0 += 64) {
@@ -957,9 +1011,9 @@ __global__ __launch_bounds__(256) void k_green(GreenArgs a) {
         [&](int k, int c) {
           if (k >= n2 || c >= n2) return 0.0;
           const int j = c >> 1, comp = c & 1;
-          if (k < n) { const cplx v = Sm[k * GN_MAX + j]; return comp ? v.im : v.re; }
-          const cplx v = Sm[(k - n) * GN_MAX + j];
-          return comp ? v.re : -v.im;
+          if (k < n) return Si[k * SG_LD + 2 * j + comp];
+          const double* v = Si + (k - n) * SG_LD + 2 * j;
+          return comp ? v[0] : -v[1];
         },
         n2, stage);
     e.store([&](int r, int c, double v) {
@@ -973,10 +1027,7 @@ __global__ __launch_bounds__(256) void k_green(GreenArgs a) {
   }
   e1re = block_sum(e1re, red);
   e1im = block_sum(e1im, red);
-  if (tid == 0) {
-    if (nswap & 1) ld_im += M_PI;
-    a.logdet[2 * ((int64_t)w * a.nslots + si)] = ld_re;
-    a.logdet[2 * ((int64_t)w * a.nslots + si) + 1] = ld_im;
+  if (threadIdx.x == 0) {
     a.e1[2 * ((int64_t)w * a.nslots + si)] = e1re;
     a.e1[2 * ((int64_t)w * a.nslots + si) + 1] = e1im;
   }
@@ -984,11 +1035,16 @@ __global__ __launch_bounds__(256) void k_green(GreenArgs a) {
 
 void launch_green(const DevState& s, bool all, hipStream_t st) {
   GreenArgs a;
-  a.psi = s.psi; a.theta = s.theta; a.logdet = s.logdet; a.e1 = s.e1; a.weights = s.weights;
+  a.psi = s.psi; a.theta = s.theta; a.logdet = s.logdet; a.e1 = s.e1; a.weights = s.weights; a.Sg = s.Sg;
   a.phi0 = s.phi[0]; a.phi1 = s.phi[1]; a.hphi0 = s.hphi[0]; a.hphi1 = s.hphi[1];
   a.ms = s.ms; a.Ktot = s.Ktot; a.det_off0 = s.det_off[0]; a.det_off1 = s.det_off[1];
-  a.N = s.N; a.nslots = s.nslots; a.s0 = s.slot[0]; a.s1 = s.slot[s.nslots > 1 ? 1 : 0]; a.all = all;
-  hipLaunchKernelGGL(k_green, dim3(s.W, s.nslots), dim3(256), 0, st, a);
+  a.N = s.N; a.nslots = s.nslots; a.W = s.W; a.s0 = s.slot[0]; a.s1 = s.slot[s.nslots > 1 ? 1 : 0]; a.all = all;
+  for (int si = 0; si < s.nslots; ++si) {
+    const int gx = (int)cdivu((int64_t)s.W * 2 * s.slot[si].n, 128);
+    hipLaunchKernelGGL(k_overlap, dim3(gx), dim3(256), 0, st, a, si, gx);
+  }
+  hipLaunchKernelGGL(k_invert, dim3(s.W, s.nslots), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_theta, dim3(s.W, s.nslots), dim3(256), 0, st, a);
 }
 
 // ------------------------------------------------------------------ weights
