@@ -290,24 +290,29 @@ struct MmaBF16 {
     uint4 ra[A_PER_T], rb[B_PER_T];
     const u16* pA[A_PER_T];
     const u16* pB[B_PER_T];
+    int oA[A_PER_T], oB[B_PER_T];   // LDS offsets of the chunks this thread stages
 #pragma unroll
     for (int q = 0; q < A_PER_T; ++q) {
       const int e = tid + q * NT;
       pA[q] = nullptr;
+      oA[q] = 0;
       if (e < A_CH) {
         const int ch = e % CPR, r = (e / CPR) % BM, c = (e / CPR) / BM;
         const u16* p = pa(c, r);
         if (p) pA[q] = p + ch * 8;
+        oA[q] = (c * BM + r) * LDK + ch * 8;
       }
     }
 #pragma unroll
     for (int q = 0; q < B_PER_T; ++q) {
       const int e = tid + q * NT;
       pB[q] = nullptr;
+      oB[q] = 0;
       if (e < B_CH) {
         const int ch = e % CPR, r = (e / CPR) % BN, c = (e / CPR) / BN;
         const u16* p = pb(c, r);
         if (p) pB[q] = p + ch * 8;
+        oB[q] = (c * BN + r) * LDK + ch * 8;
       }
     }
     const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
@@ -330,21 +335,11 @@ struct MmaBF16 {
     if (nst > 0) AFQ_BF16_FETCH(0)
     for (int s = 0; s < nst; ++s) {
 #pragma unroll
-      for (int q = 0; q < A_PER_T; ++q) {
-        const int e = tid + q * NT;
-        if (e < A_CH) {
-          const int ch = e % CPR, r = (e / CPR) % BM, c = (e / CPR) / BM;
-          *reinterpret_cast<uint4*>(As + (c * BM + r) * LDK + ch * 8) = ra[q];
-        }
-      }
+      for (int q = 0; q < A_PER_T; ++q)
+        if (A_CH % NT == 0 || tid + q * NT < A_CH) *reinterpret_cast<uint4*>(As + oA[q]) = ra[q];
 #pragma unroll
-      for (int q = 0; q < B_PER_T; ++q) {
-        const int e = tid + q * NT;
-        if (e < B_CH) {
-          const int ch = e % CPR, r = (e / CPR) % BN, c = (e / CPR) / BN;
-          *reinterpret_cast<uint4*>(Bs + (c * BN + r) * LDK + ch * 8) = rb[q];
-        }
-      }
+      for (int q = 0; q < B_PER_T; ++q)
+        if (B_CH % NT == 0 || tid + q * NT < B_CH) *reinterpret_cast<uint4*>(Bs + oB[q]) = rb[q];
       __syncthreads();
       if (s + 1 < nst) AFQ_BF16_FETCH((s + 1) * BK)
 #pragma unroll

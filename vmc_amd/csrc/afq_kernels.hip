@@ -18,6 +18,7 @@
 
 #include "afq_device.hpp"
 #include "afq_internal.hpp"
+#include <type_traits>
 
 namespace afq {
 
@@ -1478,6 +1479,39 @@ struct XPairs {
   }
 };
 
+// The same pair sum over a whole tile of f held in LDS as FP32 at once: rows gl*n + i
+// (g-block gl of the tile), columns wl*2n + 2j + {re, im} (walker wl), an even row
+// pitch so (re, im) reads are 8-byte.  A walker's tpw threads split its gt*n rows
+// ("units"); tpu threads share a unit and stride its n columns.  Per unit the inner
+// sum runs in ACC (float for the one-pass surrogate, double otherwise).
+template <class ACC>
+struct XPairsF : XPairs {
+  __device__ XPairsF(int wt) : XPairs(wt) {}
+  __device__ __forceinline__ void add(const float* F, int LDF, int n, int gt, int wt, double& accr,
+                                      double& acci) const {
+    if (wl >= wt) return;
+    const float* Fw = F + wl * 2 * n;
+    const int units = gt * n;
+    int tpu = 1;
+    while (tpu * 2 * units <= tpw) tpu *= 2;
+    const int ustep = tpw / tpu, j0 = lt & (tpu - 1);
+    for (int u = lt / tpu; u < units; u += ustep) {
+      const int gl = u / n, i = u - gl * n;
+      const float* ri = Fw + u * LDF;
+      const float* cj = Fw + gl * n * LDF + 2 * i;
+      ACC sr = 0, si = 0;
+      for (int j = j0; j < n; j += tpu) {
+        const float2 a = *reinterpret_cast<const float2*>(ri + 2 * j);
+        const float2 b = *reinterpret_cast<const float2*>(cj + j * LDF);
+        sr += (ACC)a.x * (ACC)b.x - (ACC)a.y * (ACC)b.y;
+        si += (ACC)a.x * (ACC)b.y + (ACC)a.y * (ACC)b.x;
+      }
+      accr += (double)sr;
+      acci += (double)si;
+    }
+  }
+};
+
 constexpr int XC_BM = 128, XC_BN = 128;
 int exchange_tile_g(int n) { return (n >= 1 && n <= XC_BN / 2) ? XC_BM / n : 0; }
 int exchange_tile_w(int n) { return (n >= 1 && n <= XC_BN / 2) ? XC_BN / (2 * n) : 0; }
@@ -1633,8 +1667,11 @@ __global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u
                                                           int G, Slot s0, Slot s1, int ctot, int ngt, int gridx,
                                                           int gridy, int gridz) {
   using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, 64>;
-  constexpr int LDF = XC_BN + 1;
-  constexpr int FT_BYTES = (XC_BN / 2) * LDF * 4;
+  // f of the whole tile at once when it fits in the engine's LDS (NS = 2); else one
+  // g-block at a time, which keeps the one-pass kernel at three workgroups per CU
+  constexpr int LDF = XC_BN + 2;
+  constexpr bool ONE_PASS = XC_BM * LDF * 4 <= Eng::LDS_BYTES;
+  constexpr int FT_BYTES = (ONE_PASS ? XC_BM : XC_BM / 2) * LDF * 4;
   constexpr int LDSB = Eng::LDS_BYTES > FT_BYTES ? Eng::LDS_BYTES : FT_BYTES;
   __shared__ __attribute__((aligned(16))) unsigned char lds[LDSB];
   __shared__ double red[8];
@@ -1659,18 +1696,28 @@ __global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u
         return (c ? Tl : Th) + (((int64_t)w * 2 + (cc & 1)) * (ctot / 2) + coff / 2 + (cc >> 1)) * Np;
       },
       Np, lds);
+  // g-blocks past GE hold zeros (their A rows are zero)
   float* F = reinterpret_cast<float*>(lds);
-  const XPairs xp(wt);
+  const XPairsF<typename std::conditional<NS == 1, float, double>::type> xp(wt);
   double accr = 0.0, acci = 0.0;
-  for (int gl = 0; gl < gt; ++gl) {
-    if (g0 + gl >= GE) break;
+  if (ONE_PASS) {
+    const int rows = gt * n;
     e.store([&](int r, int c, float v) {
-      const int rr = r - gl * n;
-      if (rr >= 0 && rr < n) F[rr * LDF + c] = v;
+      if (r < rows) F[r * LDF + c] = v;
     });
     __syncthreads();
-    xp.add(F, LDF, n, wt, accr, acci);
-    __syncthreads();
+    xp.add(F, LDF, n, gt, wt, accr, acci);
+  } else {
+    for (int gl = 0; gl < gt; ++gl) {
+      if (g0 + gl >= GE) break;
+      e.store([&](int r, int c, float v) {
+        const int rr = r - gl * n;
+        if (rr >= 0 && rr < n) F[rr * LDF + c] = v;
+      });
+      __syncthreads();
+      xp.add(F, LDF, n, 1, wt, accr, acci);
+      __syncthreads();
+    }
   }
   if (xp.finish(accr, acci, red) && xp.wl < wt && w0 + xp.wl < nw) {
     const int w = sel ? sel[w0 + xp.wl] : w0 + xp.wl;
