@@ -178,6 +178,7 @@ void free_all(DevState& s) {
   for (double* p : ptrs)
     if (p) (void)hipFree(p);
   if (s.idx) (void)hipFree(s.idx);
+  if (s.vhs_blk) (void)hipFree(s.vhs_blk);
   if (s.ncap) (void)hipFree(s.ncap);
   unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo, s.Lt_hi, s.Lt_lo, s.Fb_hi, s.Fb_lo,
                           s.Vh, s.Vl, s.Ub[0], s.Ub[1], s.Ub[2], s.Dh, s.Dl};
@@ -206,6 +207,16 @@ void ensure_vhs_lowprec(DevState& s, hipStream_t st) {
   s.Lt_lo = dalloc<unsigned short>(s.T * s.Gp);
   s.Fb_hi = dalloc<unsigned short>((int64_t)s.W * 2 * s.Gp);
   s.Fb_lo = dalloc<unsigned short>((int64_t)s.W * 2 * s.Gp);
+  std::vector<int> blk;
+  for (int i0 = 0; i0 < s.N; i0 += 16)
+    for (int j0 = 0; j0 < s.N && j0 <= i0 + 15; j0 += 8) {
+      blk.push_back(i0);
+      blk.push_back(j0);
+    }
+  s.nvhs_blk = (int)blk.size() / 2;
+  s.vhs_blk = dalloc<int>(blk.size());
+  hip_check(hipMemcpyAsync(s.vhs_blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st), "H2D");
+  hip_check(hipStreamSynchronize(st), "sync");
   afq::launch_lt_split(s, st);
   launch_check("lt_split");
 }
@@ -250,6 +261,7 @@ void free_temporary(const DevState& t) {
   z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
   z.Rb_hi = z.Rb_lo = z.Tb_hi = z.Tb_lo = nullptr;
   z.Lt_hi = z.Lt_lo = z.Fb_hi = z.Fb_lo = nullptr;
+  z.vhs_blk = nullptr;
   z.Vh = z.Vl = z.Ub[0] = z.Ub[1] = z.Ub[2] = z.Dh = z.Dl = nullptr;
   free_all(z);
 }

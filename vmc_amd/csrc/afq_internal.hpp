@@ -107,6 +107,8 @@ struct DevState {
   int Gp = 0;               // G padded to a multiple of 8
   unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
   unsigned short* Fb_hi = nullptr; unsigned short* Fb_lo = nullptr;  // fields, [w][re/im][Gp]
+  int* vhs_blk = nullptr;   // (i0, j0) of the 16 x 8 blocks of V that touch the lower triangle
+  int nvhs_blk = 0;
 };
 
 // Exchange-energy surrogate: GELU MLP dims[0] -> dims[1] -> dims[2] -> dims[3] -> 1,

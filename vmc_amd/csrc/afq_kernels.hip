@@ -1862,18 +1862,22 @@ __global__ void k_fields_split(const double* fields, u16* hi, u16* lo, int W, in
   }
 }
 
-// P[w][c] (both triangles) = sum_g field[w][g][c] L_g from the split operands.  With
-// Vh != nullptr the result goes straight to the split [-P_im | P_re] chunks that the
-// split-BF16 Taylor series reads (Vh/Vl[w][chunk][i][32]) instead of the FP64 P.
+// P[w][c] (both triangles) = sum_g field[w][g][c] L_g from the split operands.  A
+// tile's 128 columns are a 16 x 8 block of V (rows i0.., orbitals j0..; blk lists the
+// blocks that touch the lower triangle), each column reading the packed row of L_ij
+// for j <= i, so the epilogue knows (i, j) without inverting the packed index and
+// writes runs of 8 orbitals.  With Vh != nullptr the result goes straight to the split
+// [-P_im | P_re] chunks that the split-BF16 Taylor series reads (Vh/Vl[w][chunk][i][32])
+// instead of the FP64 P.
 template <int NS>
 __global__ __launch_bounds__(256, 2) void k_vhs_bf16(const u16* Fh, const u16* Fl, const u16* Lh, const u16* Ll,
-                                                     double* P, const double* weights, int W, int64_t T, int N,
+                                                     double* P, const double* weights, int W, const int2* blk, int N,
                                                      int Gp, int gridx, int gridy, u16* Vh, u16* Vl, int Nq) {
   using Eng = MmaBF16<128, 128, 2, 2, NS, NS>;
   __shared__ __attribute__((aligned(16))) unsigned char lds[Eng::LDS_BYTES];
   const TileId tl = tile_id(gridx, gridy, 1, 8);
   const int row0 = tl.y * 128;
-  const int64_t t0 = (int64_t)tl.x * 128;
+  const int2 b0 = blk[tl.x];
   bool any = false;
   for (int q = row0; q < min(row0 + 128, 2 * W); q += 2) any |= weights[q >> 1] > 1e-8;
   if (!any) return;
@@ -1884,18 +1888,15 @@ __global__ __launch_bounds__(256, 2) void k_vhs_bf16(const u16* Fh, const u16* F
         return q < 2 * W ? (c ? Fl : Fh) + (int64_t)q * Gp : nullptr;
       },
       [&](int c, int col) -> const u16* {
-        const int64_t t = t0 + col;
-        return t < T ? (c ? Ll : Lh) + t * Gp : nullptr;
+        const int i = b0.x + (col >> 3), j = b0.y + (col & 7);
+        if (i >= N || j > i) return nullptr;
+        return (c ? Ll : Lh) + ((int64_t)i * (i + 1) / 2 + j) * Gp;
       },
       Gp, lds);
   e.store([&](int r, int c, float v) {
     const int q = row0 + r;
-    const int64_t t = t0 + c;
-    if (q >= 2 * W || t >= T) return;
-    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    if ((int64_t)(i + 1) * (i + 2) / 2 <= t) ++i;
-    if ((int64_t)i * (i + 1) / 2 > t) --i;
-    const int j = (int)(t - (int64_t)i * (i + 1) / 2);
+    const int i = b0.x + (c >> 3), j = b0.y + (c & 7);
+    if (q >= 2 * W || i >= N || j > i) return;
     if (Vh) {
       const int w = q >> 1, comp = q & 1;
       const float x = comp ? -v : v;
@@ -1918,15 +1919,16 @@ void launch_vhs_lp(const DevState& s, int prec, hipStream_t st) {
   const int64_t tot = (int64_t)s.W * 2 * s.Gp;
   hipLaunchKernelGGL(k_fields_split, dim3((unsigned)std::min<int64_t>(cdivu(tot, 256), 65535)), dim3(256), 0, st,
                      s.fields, s.Fb_hi, s.Fb_lo, s.W, s.G, s.Gp);
-  const int gx = (int)cdivu(s.T, 128), gy = (int)cdivu(2 * (int64_t)s.W, 128);
+  const int gx = s.nvhs_blk, gy = (int)cdivu(2 * (int64_t)s.W, 128);
   u16* vh = s.prec_prop != AFQ_PREC_FP64 ? s.Vh : nullptr;
   u16* vl = s.prec_prop != AFQ_PREC_FP64 ? s.Vl : nullptr;
+  const int2* blk = reinterpret_cast<const int2*>(s.vhs_blk);
   if (prec == AFQ_PREC_BF16X3)
     hipLaunchKernelGGL(k_vhs_bf16<2>, dim3(gx * gy), dim3(256), 0, st, s.Fb_hi, s.Fb_lo, s.Lt_hi, s.Lt_lo, s.P,
-                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy, vh, vl, s.Nq);
+                       s.weights, s.W, blk, s.N, s.Gp, gx, gy, vh, vl, s.Nq);
   else
     hipLaunchKernelGGL(k_vhs_bf16<1>, dim3(gx * gy), dim3(256), 0, st, s.Fb_hi, s.Fb_lo, s.Lt_hi, s.Lt_lo, s.P,
-                       s.weights, s.W, s.T, s.N, s.Gp, gx, gy, vh, vl, s.Nq);
+                       s.weights, s.W, blk, s.N, s.Gp, gx, gy, vh, vl, s.Nq);
 }
 
 // E_L = ecore + sum_s m_s e1_s + 1/2 sum_{g<GE} fb_g^2 - 1/2 sum_s m_s X_s
