@@ -69,7 +69,7 @@ def flops_per_walker_step(spec, ortho=5, energy=True):
 
 
 from vmc_amd.afqmc import STAGES  # noqa: E402
-PEAKS = {"fp64": 78.6, "bf16": 2500.0}
+PEAKS = {"fp64": 78.6, "bf16": 2500.0, "fp8": 5000.0}
 
 
 def run_path(ham, spec, total_steps, warmup, **kw):
@@ -140,7 +140,13 @@ def stage_rooflines(spec, path, n_exact):
         add("exchange", "k_exchange_bf16 (3-pass split BF16 MFMA)", exchange_flops(spec, n_exact),
             4.0 * G * ncol * Np + 8.0 * ncol * Np * n_exact, 1, "bf16", symbol="void k_exchange_bf16<2>",
             passes=3, walkers=n_exact)
-    if path.get("surrogate_prec"):
+    if path.get("surrogate_prec") == "fp8":
+        nl = W * live
+        Npq = (N + 15) // 16 * 16
+        add("exchange_lp", "k_exchange_q8 (e4m3 MFMA, per-Cholesky-vector scale, surrogate level)",
+            exchange_flops(spec, nl), G * ncol * Npq + 2 * ncol * Npq * nl, 1, "fp8", symbol="k_exchange_q8",
+            walkers=nl)
+    elif path.get("surrogate_prec"):
         passes = 3 if path["surrogate_prec"] == "bf16x3" else 1
         nl = W * live
         add("exchange_lp", "k_exchange_bf16 (%d-pass BF16 MFMA, surrogate level)" % passes,
@@ -248,10 +254,11 @@ def cpu_baseline(ham, spec, threads):
 def micro(kind, workload, nwalk=1024, calls=8):
     """Kernel microbenchmark for profiling: one context of `nwalk` walkers near the
     trial, `calls` measurements with the exchange in the given precision."""
-    from vmc_amd.abi import AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64
+    from vmc_amd.abi import AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64, AFQ_PREC_FP8
     from vmc_amd.afqmc import Context
     from vmc_amd.synthetic import workload as get_workload
-    prec = {"exchange-bf16": AFQ_PREC_BF16, "exchange-bf16x3": AFQ_PREC_BF16X3, "exchange-fp64": AFQ_PREC_FP64}[kind]
+    prec = {"exchange-bf16": AFQ_PREC_BF16, "exchange-bf16x3": AFQ_PREC_BF16X3, "exchange-fp64": AFQ_PREC_FP64,
+            "exchange-fp8": AFQ_PREC_FP8}[kind]
     h, spec = get_workload(workload)
     ctx = Context(h, nwalk, spec["walker_rhf"])
     ctx.prep_prop(h.phi_up, h.phi_dn, spec["dt"])
@@ -364,7 +371,7 @@ def main():
                          "always measured exactly")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-surrogate path")
-    ap.add_argument("--surrogate-prec", choices=["bf16", "bf16x3"], default="bf16",
+    ap.add_argument("--surrogate-prec", choices=["bf16", "bf16x3", "fp8"], default="bf16",
                     help="precision of the exchange surrogate level of the control variate")
     ap.add_argument("--fp64-propagation", action="store_true",
                     help="mixed path: keep the force bias, VHS and Taylor products in FP64 "
@@ -373,7 +380,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 1 GiB plain-copy kernel (HBM counter calibration)")
-    ap.add_argument("--micro", choices=["exchange-bf16", "exchange-bf16x3", "exchange-fp64"],
+    ap.add_argument("--micro", choices=["exchange-bf16", "exchange-bf16x3", "exchange-fp64", "exchange-fp8"],
                     help="kernel microbenchmark for profiling (1024 walkers, 8 measurements) instead of the bench")
     a = ap.parse_args()
     if a.micro:
@@ -390,7 +397,8 @@ def main():
               "--gpus %d is not supported" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
-    from vmc_amd.abi import (AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64, AFQ_SURR_EXCHANGE,
+    from vmc_amd.abi import (AFQ_ENERGY_SURROGATE, AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64, AFQ_PREC_FP8,
+                             AFQ_SURR_EXCHANGE,
                              AFQ_SURR_MLP)
     from vmc_amd.synthetic import workload
     t0 = time.time()
@@ -417,7 +425,8 @@ def main():
         xcv = run_path(ham, spec, a.warmup + a.steps, a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
                        prec_fb=pp, prec_vhs=pp, prec_prop=pp, prec_exchange=AFQ_PREC_FP64,
                        surrogate_kind=AFQ_SURR_EXCHANGE,
-                       surrogate_prec=AFQ_PREC_BF16X3 if a.surrogate_prec == "bf16x3" else AFQ_PREC_BF16,
+                       surrogate_prec={"bf16": AFQ_PREC_BF16, "bf16x3": AFQ_PREC_BF16X3,
+                                       "fp8": AFQ_PREC_FP8}[a.surrogate_prec],
                        subset_inv=a.subset_inv, risk_frac=0.0, shadow_exact=True)
         nx = W * xcv["live_frac"] / a.subset_inv
         xcv.update(exchange_prec="fp64", surrogate_prec=a.surrogate_prec, n_exact=nx,

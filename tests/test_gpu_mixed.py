@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from test_gpu_parity import build, perturbed_walkers
-from vmc_amd.abi import AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64
+from vmc_amd.abi import AFQ_PREC_BF16, AFQ_PREC_BF16X3, AFQ_PREC_FP64, AFQ_PREC_FP8
 
 pytestmark = pytest.mark.gpu
 
@@ -87,3 +87,63 @@ def test_mixed_step_vs_fp64(name, prop):
     re = np.max(np.abs(e2[live] - e1[live]) / np.abs(e1[live]))
     print(name, prop, "mixed vs fp64 after 3 steps: dets %.2e weights %.2e E_L %.2e" % (rd, rw, re))
     assert rd < 1e-5 and rw < 1e-5 and re < 1e-5
+
+
+def _e4m3(x):
+    """Round float32 values to OCP e4m3 (bias 7, largest 448), nearest even, saturating."""
+    a = np.abs(x.astype(np.float64))
+    out = np.empty_like(a)
+    sub = a < 2.0 ** -6
+    out[sub] = np.rint(a[sub] * 512.0) / 512.0
+    m, e = np.frexp(a[~sub])            # a = m 2^e, m in [0.5, 1): quantum 2^(e - 4)
+    q = np.ldexp(1.0, e - 4)
+    out[~sub] = np.rint(a[~sub] / q) * q
+    return np.sign(x) * np.minimum(out, 448.0)
+
+
+def _e4m3_scale(amax):
+    """Power of two s with amax * s in (224, 448], from float32 arithmetic."""
+    amax = np.float32(amax)
+    if not amax > 1e-30:
+        return 1.0
+    _, e = np.frexp(np.float32(448.0) / amax)
+    return float(np.ldexp(1.0, int(e) - 1))
+
+
+def _quant(x, s):
+    return _e4m3((x * s).astype(np.float32)) / s
+
+
+@pytest.mark.parametrize("name", ["uhf_33_31", "uhf_n60", "c1_small"])
+def test_exchange_fp8_matches_quantized_model(name):
+    """e4m3 exchange: each Cholesky vector's rotated rows scaled by a power of two
+    (largest entry over both spins into (224, 448]), each walker's theta of a spin
+    likewise, round to nearest even, FP32 accumulation.  The GPU energies equal the
+    FP64 energies shifted by -1/2 (X_q - X) computed in numpy from the quantized
+    operands, to 2% of that shift (a wrong format, bias or scale is off by O(1))."""
+    W = 5
+    h, wrhf, ctx, orc = build(name, W)
+    dets = perturbed_walkers(h, W, wrhf, np.random.default_rng(15), 0.05)
+    ctx.set_walkers(dets)
+    e64, _ = ctx.measure()
+    ctx.set_precision(prec_exchange=AFQ_PREC_FP8)
+    e8, _ = ctx.measure()
+    phis = [h.phi_up, h.phi_dn]
+    R = [np.einsum("ia,gij->gaj", p, h.chol) for p in phis]            # G x n x N
+    amax = [max(np.float32(np.abs(R[0][g]).max()), np.float32(np.abs(R[1][g]).max())) for g in range(h.nchol)]
+    sg = np.array([_e4m3_scale(a) for a in amax])
+    Rq = [np.stack([_quant(R[s][g], sg[g]) for g in range(h.nchol)]) for s in range(2)]
+    pred = []
+    for w in range(W):
+        dx = 0.0
+        for s, psi in enumerate(h.split_det(dets[w])):
+            th = psi @ np.linalg.inv(phis[s].T @ psi)                    # N x n
+            t = _e4m3_scale(max(np.abs(th.real).max(), np.abs(th.imag).max()))
+            thq = _quant(th.real, t) + 1j * _quant(th.imag, t)
+            f = np.einsum("gak,kb->gab", R[s], th)
+            fq = np.einsum("gak,kb->gab", Rq[s], thq)
+            dx += np.einsum("gab,gba->", fq, fq) - np.einsum("gab,gba->", f, f)
+        pred.append(-0.5 * dx.real)
+    pred = np.array(pred)
+    print(name, "fp8 shift of E_L: gpu", e8 - e64, "model", pred)
+    assert np.all(np.abs((e8 - e64) - pred) <= 0.02 * np.abs(pred) + 1e-9 * np.abs(e64))

@@ -184,6 +184,9 @@ void free_all(DevState& s) {
                           s.Vh, s.Vl, s.Ub[0], s.Ub[1], s.Ub[2], s.Dh, s.Dl};
   for (unsigned short* p : hp)
     if (p) (void)hipFree(p);
+  void* vp[] = {s.Rq8, s.Tq8, s.rinv8, s.tinv8};
+  for (void* p : vp)
+    if (p) (void)hipFree(p);
 }
 
 // bf16 operand copies for the low-precision exchange (allocated on first use)
@@ -251,6 +254,27 @@ void run_force_bias(const DevState& s, hipStream_t st) {
 bool exchange_prec_ok(int p) {
   return p == AFQ_PREC_FP64 || p == AFQ_PREC_BF16X3 || p == AFQ_PREC_BF16;
 }
+// the exchange contraction also runs in e4m3
+bool exchange_only_prec_ok(int p) { return exchange_prec_ok(p) || p == AFQ_PREC_FP8; }
+
+// e4m3 exchange operands (allocated on first use; the rotated Cholesky is converted once)
+void ensure_q8(DevState& s, hipStream_t st) {
+  if (s.Rq8) return;
+  s.Npq = (s.N + 15) / 16 * 16;
+  int64_t rows = 0;
+  for (int q = 0; q < s.nslots; ++q) rows += (int64_t)s.G * s.slot[q].n;
+  s.Rq8 = dalloc<unsigned char>(rows * s.Npq);
+  s.rinv8 = dalloc<float>(s.G);
+  s.Tq8 = dalloc<unsigned char>((int64_t)s.W * s.ctot * s.Npq);
+  s.tinv8 = dalloc<float>(2 * (int64_t)s.W);
+  afq::launch_rq8(s, st);
+  launch_check("rq8");
+}
+// operand buffers of an exchange precision
+void ensure_exchange_prec(DevState& s, int prec, hipStream_t st) {
+  if (prec == AFQ_PREC_FP8) ensure_q8(s, st);
+  else if (prec != AFQ_PREC_FP64) ensure_lowprec(s, st);
+}
 
 // Frees the walker buffers of a temporary population that shares the
 // Hamiltonian, trial and bf16 operand buffers of its context.
@@ -262,6 +286,8 @@ void free_temporary(const DevState& t) {
   z.Rb_hi = z.Rb_lo = z.Tb_hi = z.Tb_lo = nullptr;
   z.Lt_hi = z.Lt_lo = z.Fb_hi = z.Fb_lo = nullptr;
   z.vhs_blk = nullptr;
+  z.Rq8 = nullptr; z.rinv8 = nullptr;   // shared with the context (Tq8/tinv8 are sized by its W)
+  z.Tq8 = nullptr; z.tinv8 = nullptr;
   z.Vh = z.Vl = z.Ub[0] = z.Ub[1] = z.Ub[2] = z.Dh = z.Dl = nullptr;
   free_all(z);
 }
@@ -495,7 +521,7 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
   if (na > 64 || nb > 64)
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "more than 64 electrons per spin not supported by this build");
   if (!exchange_prec_ok(params->prec_fb) || !exchange_prec_ok(params->prec_vhs) ||
-      !exchange_prec_ok(params->prec_exchange))
+      !exchange_only_prec_ok(params->prec_exchange))
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
   if (params->ref_compat && params->prec_vhs != AFQ_PREC_FP64)
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "ref_compat rounds the VHS itself; it needs prec_vhs = FP64");
@@ -575,7 +601,8 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     s.mf = dalloc<double>(2 * (size_t)G);
     alloc_walkers(s);
     afq::launch_fill(s.weights, W, 0.0, c->stream);
-    if (s.prec_exchange != AFQ_PREC_FP64 || s.prec_fb != AFQ_PREC_FP64) ensure_lowprec(s, c->stream);
+    ensure_exchange_prec(s, s.prec_exchange, c->stream);
+    if (s.prec_fb != AFQ_PREC_FP64) ensure_lowprec(s, c->stream);
     if (s.prec_vhs != AFQ_PREC_FP64) ensure_vhs_lowprec(s, c->stream);
     hip_check(hipStreamSynchronize(c->stream), "setup sync");
   } catch (const DevError& e) {
@@ -606,12 +633,13 @@ int afq_destroy(afq_ctx* c) {
 
 int afq_set_precision(afq_ctx* c, int32_t prec_fb, int32_t prec_vhs, int32_t prec_exchange) {
   if (!c) return AFQ_ERR_ARG;
-  if (!exchange_prec_ok(prec_fb) || !exchange_prec_ok(prec_vhs) || !exchange_prec_ok(prec_exchange))
+  if (!exchange_prec_ok(prec_fb) || !exchange_prec_ok(prec_vhs) || !exchange_only_prec_ok(prec_exchange))
     return fail(c, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
   if (c->s.ref_compat && prec_vhs != AFQ_PREC_FP64)
     return fail(c, AFQ_ERR_UNSUPPORTED, "ref_compat rounds the VHS itself; it needs prec_vhs = FP64");
   AFQ_GUARD_BEGIN
-  if (prec_exchange != AFQ_PREC_FP64 || prec_fb != AFQ_PREC_FP64) ensure_lowprec(c->s, c->stream);
+  ensure_exchange_prec(c->s, prec_exchange, c->stream);
+  if (prec_fb != AFQ_PREC_FP64) ensure_lowprec(c->s, c->stream);
   if (prec_vhs != AFQ_PREC_FP64) ensure_vhs_lowprec(c->s, c->stream);
   c->s.prec_exchange = prec_exchange;
   c->s.prec_fb = prec_fb;
@@ -1441,10 +1469,10 @@ int afq_set_cv(afq_ctx* c, int32_t subset_inv, double risk_frac, double fallback
 int afq_set_surrogate_kind(afq_ctx* c, int32_t kind, int32_t prec) {
   if (!c) return AFQ_ERR_ARG;
   if (kind != AFQ_SURR_MLP && kind != AFQ_SURR_EXCHANGE) return fail(c, AFQ_ERR_ARG, "unknown surrogate kind");
-  if (kind == AFQ_SURR_EXCHANGE && prec != AFQ_PREC_BF16 && prec != AFQ_PREC_BF16X3)
-    return fail(c, AFQ_ERR_UNSUPPORTED, "surrogate exchange precision must be BF16 or BF16X3");
+  if (kind == AFQ_SURR_EXCHANGE && prec != AFQ_PREC_BF16 && prec != AFQ_PREC_BF16X3 && prec != AFQ_PREC_FP8)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "surrogate exchange precision must be BF16, BF16X3 or FP8");
   AFQ_GUARD_BEGIN
-  if (kind == AFQ_SURR_EXCHANGE) ensure_lowprec(c->s, c->stream);
+  if (kind == AFQ_SURR_EXCHANGE) ensure_exchange_prec(c->s, prec, c->stream);
   ensure_cv(c);
   c->cv_kind = kind;
   c->cv_prec = prec;

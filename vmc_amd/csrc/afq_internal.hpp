@@ -107,6 +107,11 @@ struct DevState {
   int Gp = 0;               // G padded to a multiple of 8
   unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
   unsigned short* Fb_hi = nullptr; unsigned short* Fb_lo = nullptr;  // fields, [w][re/im][Gp]
+  // e4m3 exchange operands (allocated on first use): Rq8[slot][g][i][Npq] with 1/s_g in
+  // rinv8[g]; Tq8[w][re/im][slot][j][Npq] with 1/t in tinv8[w][slot]; Npq = N padded to 16
+  int Npq = 0;
+  unsigned char* Rq8 = nullptr; unsigned char* Tq8 = nullptr;
+  float* rinv8 = nullptr; float* tinv8 = nullptr;
   int* vhs_blk = nullptr;   // (i0, j0) of the 16 x 8 blocks of V that touch the lower triangle
   int nvhs_blk = 0;
 };
@@ -161,6 +166,7 @@ void launch_energy(const DevState& s, const int* sel, int nsel, double delta, hi
 void launch_comb(DevState& s, double zeta, hipStream_t st);
 inline int onebody_pitch(int N) { return (N + 31) / 32 * 32; }
 void launch_v_split(const DevState& s, hipStream_t st);
+void launch_rq8(const DevState& s, hipStream_t st);
 void launch_d_split(const DevState& s, hipStream_t st);
 void launch_copy_det0_to_det1(const DevState& s, double* dst, const double* src, hipStream_t st);
 

@@ -1727,7 +1727,12 @@ __global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u
   }
 }
 
+static void launch_exchange_q8(const DevState& s, const int* sel, int nsel, hipStream_t st);
 void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, hipStream_t st) {
+  if (prec == AFQ_PREC_FP8) {
+    launch_exchange_q8(s, sel, nsel, st);
+    return;
+  }
   launch_theta_split(s, sel, nsel, st);
   auto go = [&](int z0, int nz, double* xp) {
     const Slot& a = s.slot[z0];
@@ -1762,6 +1767,243 @@ void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, h
     else
       hipLaunchKernelGGL(k_exchange_bf16<1>, dim3(ngt * gy), dim3(256), 0, st, rh, rl, th, tlo, sel, xp, s.Np, s.GE,
                          nsel, s.W, s.G, bb, bb, s.ctot, s.ngt_cap, ngt, gy, 1);
+  }
+}
+
+// ------------------------------------------------------------------ FP8 (e4m3) exchange
+// Operands in OCP e4m3 (1-4-3, bias 7, largest finite 448), each Cholesky vector g
+// scaled by a power of two s_g (its largest |R_g| entry, over both slots, maps into
+// (224, 448]) and each walker's theta of a slot by a power of two t; the products
+// accumulate in FP32 on v_mfma_scale_f32_16x16x128_f8f6f4 (unit block scales) and
+// the epilogue multiplies f by 1/(s_g t), exactly, before the pair sums.
+
+// float -> e4m3, round to nearest even, saturating at 448
+__device__ __forceinline__ unsigned f2e4m3(float x) {
+  const unsigned sgn = (__float_as_uint(x) >> 24) & 0x80u;
+  const float a = fabsf(x);
+  if (!(a < 464.f)) return sgn | 0x7eu;
+  if (a < 0.015625f) return sgn | (unsigned)rintf(a * 512.f);   // subnormals: multiples of 2^-9 (8 = 2^-6)
+  const unsigned u = __float_as_uint(a);
+  int e = (int)(u >> 23) - 127;
+  const unsigned m = u & 0x7fffffu, rem = m & 0xfffffu;
+  unsigned r = m >> 20;
+  if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) ++r;
+  if (r == 8u) { r = 0u; ++e; }
+  if (e > 8 || (e == 8 && r == 7u)) { e = 8; r = 6u; }
+  return sgn | ((unsigned)(e + 7) << 3) | r;
+}
+// the power of two s with amax * s in (224, 448] (1 for amax = 0)
+__device__ __forceinline__ float e4m3_scale(float amax) {
+  if (!(amax > 1e-30f)) return 1.f;
+  int e;
+  frexpf(448.f / amax, &e);   // 448/amax = f 2^e, f in [0.5, 1)
+  return ldexpf(1.f, e - 1);
+}
+__device__ __forceinline__ float block_max(float v, float* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float m = 0.f;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) m = fmaxf(m, red[q]);
+  __syncthreads();
+  return m;
+}
+
+// Rq[slot][g][i][Npq] = e4m3(s_g R_s,g(i, k)), rinv[g] = 1 / s_g; one workgroup per g
+__global__ __launch_bounds__(256) void k_rq8(const double* R, unsigned char* Rq, float* rinv, int N, int Npq, int G,
+                                             int64_t Ktot, Slot s0, Slot s1, int nslots) {
+  __shared__ float red[4];
+  const int g = blockIdx.x;
+  const double* Rg = R + (int64_t)g * Ktot;
+  float amax = 0.f;
+  for (int q = 0; q < nslots; ++q) {
+    const Slot sl = q ? s1 : s0;
+    for (int e = threadIdx.x; e < sl.n * N; e += blockDim.x) amax = fmaxf(amax, (float)fabs(Rg[sl.koff + e]));
+  }
+  const float sc = e4m3_scale(block_max(amax, red));
+  if (threadIdx.x == 0) rinv[g] = 1.f / sc;
+  int64_t off = 0;
+  for (int q = 0; q < nslots; ++q) {
+    const Slot sl = q ? s1 : s0;
+    unsigned char* dst = Rq + off + (int64_t)g * sl.n * Npq;
+    for (int e = threadIdx.x; e < sl.n * Npq; e += blockDim.x) {
+      const int i = e / Npq, k = e - i * Npq;
+      dst[e] = k < N ? (unsigned char)f2e4m3((float)(Rg[sl.koff + (int64_t)k * sl.n + i] * (double)sc)) : 0;
+    }
+    off += (int64_t)G * sl.n * Npq;
+  }
+}
+
+// Tq[w][re/im][slot][j][Npq] = e4m3(t theta_s[w][k][j]), tinv[w][slot] = 1 / t; one
+// workgroup per (selected walker, slot)
+__global__ __launch_bounds__(256) void k_theta_q8(const double* theta, const int* sel, unsigned char* Tq, float* tinv,
+                                                  int N, int Npq, int64_t Ktot, Slot s0, Slot s1, int ctot) {
+  __shared__ float red[4];
+  const int w = sel ? sel[blockIdx.x] : blockIdx.x;
+  const Slot sl = blockIdx.y == 0 ? s0 : s1;
+  const int n2 = 2 * sl.n;
+  const int coff = blockIdx.y == 0 ? 0 : 2 * s0.n;
+  const double* th = theta + (int64_t)w * 2 * Ktot + 2 * sl.koff;
+  float amax = 0.f;
+  for (int e = threadIdx.x; e < N * n2; e += blockDim.x) amax = fmaxf(amax, (float)fabs(th[e]));
+  const float sc = e4m3_scale(block_max(amax, red));
+  if (threadIdx.x == 0) tinv[2 * (int64_t)w + blockIdx.y] = 1.f / sc;
+  for (int e = threadIdx.x; e < n2 * Npq; e += blockDim.x) {
+    const int c = e / Npq, k = e - c * Npq;
+    const int64_t a = (((int64_t)w * 2 + (c & 1)) * (ctot / 2) + coff / 2 + (c >> 1)) * Npq + k;
+    Tq[a] = k < N ? (unsigned char)f2e4m3((float)(th[(int64_t)k * n2 + c] * (double)sc)) : 0;
+  }
+}
+
+// Xp as k_exchange from the e4m3 operands: 128 x 128 tiles, four waves of 64 x 64, K
+// stages of 128 (one v_mfma_scale_f32_16x16x128_f8f6f4 deep), register-prefetched.
+// Lane l supplies 32 consecutive K bytes at 32 (l / 16) of row l % 16 for both A and
+// B, so every product pairs equal k whatever the instruction's internal K order.
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+constexpr int Q8_BK = 128, Q8_LDK = Q8_BK + 16;
+__global__ __launch_bounds__(256, 2) void k_exchange_q8(const unsigned char* Rq, const unsigned char* Tq,
+                                                        const float* rinv, const float* tinv, const int* sel,
+                                                        double* Xp, int Npq, int GE, int nw, int W, int G, Slot s0,
+                                                        Slot s1, int ctot, int ngt_cap, int gridx, int gridy,
+                                                        int gridz) {
+  constexpr int LDF = XC_BN + 2;
+  constexpr int STAGE = 2 * XC_BM * Q8_LDK, FT = (XC_BM / 2) * LDF * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[STAGE > FT ? STAGE : FT];
+  __shared__ double red[8];
+  unsigned char* As = lds;
+  unsigned char* Bs = lds + XC_BM * Q8_LDK;
+  const TileId tl = tile_id(gridx, gridy, gridz, 8);
+  const int zs = gridz > 1 ? tl.z : 0;
+  const Slot sl = zs == 0 ? s0 : s1;
+  const int n = sl.n, gt = XC_BM / n, wt = XC_BN / (2 * n);
+  const int g0 = tl.x * gt, w0 = tl.y * wt;
+  const int coff = (zs == 0 && sl.koff == s0.koff) ? 0 : 2 * s0.n;
+  const int64_t roff = (zs == 0 && sl.koff == s0.koff) ? 0 : (int64_t)G * s0.n * Npq;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
+  // 16-byte chunks: thread -> rows (tid >> 3) + 32 q, chunk (tid & 7) of the 128-byte stage
+  const int ch = tid & 7;
+  const unsigned char* pA[4];
+  const unsigned char* pB[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = (tid >> 3) + 32 * q;
+    const int gl = r / n, i = r - gl * n, g = g0 + gl;
+    pA[q] = (gl < gt && g < GE) ? Rq + roff + ((int64_t)g * n + i) * Npq + 16 * ch : nullptr;
+    const int wl = r / (2 * n), cc = r - wl * 2 * n;
+    pB[q] = nullptr;
+    if (wl < wt && w0 + wl < nw) {
+      const int w = sel ? sel[w0 + wl] : w0 + wl;
+      pB[q] = Tq + (((int64_t)w * 2 + (cc & 1)) * (ctot / 2) + coff / 2 + (cc >> 1)) * Npq + 16 * ch;
+    }
+  }
+  uint4 ra[4], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      ra[q] = uint4{0u, 0u, 0u, 0u};
+      rb[q] = uint4{0u, 0u, 0u, 0u};
+      if (k0 + 16 * ch < Npq) {
+        if (pA[q]) ra[q] = *reinterpret_cast<const uint4*>(pA[q] + k0);
+        if (pB[q]) rb[q] = *reinterpret_cast<const uint4*>(pB[q] + k0);
+      }
+    }
+  };
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nst = (Npq + Q8_BK - 1) / Q8_BK;
+  fetch(0);
+  for (int st = 0; st < nst; ++st) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = (tid >> 3) + 32 * q;
+      *reinterpret_cast<uint4*>(As + r * Q8_LDK + 16 * ch) = ra[q];
+      *reinterpret_cast<uint4*>(Bs + r * Q8_LDK + 16 * ch) = rb[q];
+    }
+    __syncthreads();
+    if (st + 1 < nst) fetch((st + 1) * Q8_BK);
+    const int koff = 32 * (lane >> 4);
+    i32x8 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const unsigned char* pa = As + (wm * 64 + i * 16 + (lane & 15)) * Q8_LDK + koff;
+      const uint4 lo = *reinterpret_cast<const uint4*>(pa), hi = *reinterpret_cast<const uint4*>(pa + 16);
+      a[i] = i32x8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+      const unsigned char* pb = Bs + (wn * 64 + i * 16 + (lane & 15)) * Q8_LDK + koff;
+      const uint4 lb = *reinterpret_cast<const uint4*>(pb), hb = *reinterpret_cast<const uint4*>(pb + 16);
+      b[i] = i32x8{(int)lb.x, (int)lb.y, (int)lb.z, (int)lb.w, (int)hb.x, (int)hb.y, (int)hb.z, (int)hb.w};
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], acc[i][j], 0, 0, 0, 127, 0, 127);
+    __syncthreads();
+  }
+  // unscale while storing f: row -> Cholesky vector g0 + gl, column -> walker w0 + c / 2n
+  float* F = reinterpret_cast<float*>(lds);
+  float cs[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int c = wn * 64 + j * 16 + (lane & 15), wl = c / (2 * n);
+    cs[j] = 0.f;
+    if (wl < wt && w0 + wl < nw) {
+      const int w = sel ? sel[w0 + wl] : w0 + wl;
+      cs[j] = tinv[2 * (int64_t)w + zs];
+    }
+  }
+  const XPairsF<float> xp(wt);
+  double accr = 0.0, acci = 0.0;
+  for (int gl = 0; gl < gt; ++gl) {
+    if (g0 + gl >= GE) break;
+    const float rs = rinv[g0 + gl];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rr = wm * 64 + i * 16 + 4 * (lane >> 4) + r - gl * n;
+          if (rr >= 0 && rr < n) F[rr * LDF + wn * 64 + j * 16 + (lane & 15)] = acc[i][j][r] * rs * cs[j];
+        }
+    __syncthreads();
+    xp.add(F, LDF, n, 1, wt, accr, acci);
+    __syncthreads();
+  }
+  if (xp.finish(accr, acci, red) && xp.wl < wt && w0 + xp.wl < nw) {
+    const int w = sel ? sel[w0 + xp.wl] : w0 + xp.wl;
+    double* o = Xp + 2 * (((int64_t)zs * W + w) * ngt_cap + tl.x);
+    o[0] = accr;
+    o[1] = acci;
+  }
+}
+
+void launch_rq8(const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_rq8, dim3(s.G), dim3(256), 0, st, s.R, s.Rq8, s.rinv8, s.N, s.Npq, s.G, s.Ktot, s.slot[0],
+                     s.slot[s.nslots > 1 ? 1 : 0], s.nslots);
+}
+
+static void launch_exchange_q8(const DevState& s, const int* sel, int nsel, hipStream_t st) {
+  hipLaunchKernelGGL(k_theta_q8, dim3(nsel, s.nslots), dim3(256), 0, st, s.theta, sel, s.Tq8, s.tinv8, s.N, s.Npq,
+                     s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ctot);
+  auto go = [&](const unsigned char* rq, const unsigned char* tq, const float* ti, double* xp, Slot a, Slot b, int nz) {
+    const int gt = exchange_tile_g(a.n), wt = exchange_tile_w(a.n);
+    const int ngt = (int)cdivu(s.GE, gt), gy = (int)cdivu(nsel, wt);
+    hipLaunchKernelGGL(k_exchange_q8, dim3(ngt * gy * nz), dim3(256), 0, st, rq, tq, s.rinv8, ti, sel, xp, s.Npq,
+                       s.GE, nsel, s.W, s.G, a, b, s.ctot, s.ngt_cap, ngt, gy, nz);
+  };
+  if (s.nslots == 1 || s.slot[1].n == s.slot[0].n) {
+    go(s.Rq8, s.Tq8, s.tinv8, s.Xp, s.slot[0], s.nslots > 1 ? s.slot[1] : s.slot[0], s.nslots);
+  } else {
+    go(s.Rq8, s.Tq8, s.tinv8, s.Xp, s.slot[0], s.slot[0], 1);
+    // the second slot alone, presented as s0 with pointers advanced to its rows and columns;
+    // its theta scales sit at tinv[2w + 1]
+    Slot bb = s.slot[1];
+    bb.koff = s.slot[0].koff;
+    go(s.Rq8 + (int64_t)s.G * s.slot[0].n * s.Npq, s.Tq8 + (int64_t)s.slot[0].n * s.Npq, s.tinv8 + 1,
+       s.Xp + 2 * (int64_t)s.W * s.ngt_cap, bb, bb, 1);
   }
 }
 
