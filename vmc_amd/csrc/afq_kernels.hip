@@ -405,18 +405,20 @@ __global__ __launch_bounds__(512) void k_taylor(const double* P, const double* t
 // = 32 real K = one v_mfma_f32_16x16x32_bf16, and its A and B slices are contiguous.
 // Intermediate u are stored split; the last step adds the FP64 t0 and writes FP64.
 // Tile: 32*MI orbital rows x 32*NI complex columns, 8 waves as 2 (rows) x 4 (columns).
+// Tile: 32*MI orbital rows x 32*NI complex columns, 8 waves as 2 (rows) x 4 (columns);
+// a stage holds CPS chunks (its loads prefetched into registers one stage ahead).
 constexpr int TH_LDK = 40;
-template <int MI, int NI>
+template <int MI, int NI, int CPS>
 __global__ __launch_bounds__(512) void k_taylor_h(const u16* Vh, const u16* Vl, const u16* Ui, u16* Uo,
                                                   const u16* T0b, const double* t0, double* out,
                                                   const double* weights, int N, int Nq, int na, int nb,
                                                   int64_t offb, int64_t wstride, float scale, int last,
                                                   int gridx, int gridy, int W) {
-  constexpr int BM = 32 * MI, BN = 64 * NI, BNC = BN / 2;
-  constexpr int AV = 2 * BM * 4, AVT = (AV + 511) / 512;     // 16-B vectors per stage: A
+  constexpr int BM = 32 * MI, BN = 64 * NI, BNC = BN / 2, LDK = 32 * CPS + 8;
+  constexpr int AV = 2 * BM * 4, AVT = (AV + 511) / 512;     // 16-B vectors per chunk: A
   constexpr int BV = 4 * BNC * 2, BVT = (BV + 511) / 512;    //                         B
-  __shared__ __attribute__((aligned(16))) u16 As[2][BM * TH_LDK];
-  __shared__ __attribute__((aligned(16))) u16 Bs[2][BN * TH_LDK];
+  __shared__ __attribute__((aligned(16))) u16 As[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][BN * LDK];
   const TileId tl = tile_id(gridx, gridy, W, gridy);
   const int w = tl.z;
   if (weights[w] <= 1e-8) return;
@@ -428,52 +430,60 @@ __global__ __launch_bounds__(512) void k_taylor_h(const u16* Vh, const u16* Vl, 
   const u16* vh = Vh + (int64_t)w * vper;
   const u16* vl = Vl + (int64_t)w * vper;
   const u16* ui = Ui + (int64_t)w * 4 * uplane;
-  uint4 ra[AVT], rb[BVT];
-  auto fetch = [&](int ch) {
+  uint4 ra[CPS][AVT], rb[CPS][BVT];
+  auto fetch = [&](int ch0) {
 #pragma unroll
-    for (int h = 0; h < AVT; ++h) {
-      const int e = tid + 512 * h;
-      ra[h] = uint4{0u, 0u, 0u, 0u};
-      if (e < AV) {
-        const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3, i = row0 + r;
-        if (i < N) ra[h] = *reinterpret_cast<const uint4*>((pl ? vl : vh) + ((int64_t)ch * N + i) * 32 + 8 * v);
+    for (int cc = 0; cc < CPS; ++cc) {
+      const int ch = ch0 + cc;
+#pragma unroll
+      for (int h = 0; h < AVT; ++h) {
+        const int e = tid + 512 * h;
+        ra[cc][h] = uint4{0u, 0u, 0u, 0u};
+        if (e < AV && ch < nch) {
+          const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3, i = row0 + r;
+          if (i < N) ra[cc][h] = *reinterpret_cast<const uint4*>((pl ? vl : vh) + ((int64_t)ch * N + i) * 32 + 8 * v);
+        }
       }
-    }
 #pragma unroll
-    for (int h = 0; h < BVT; ++h) {
-      const int e = tid + 512 * h;
-      rb[h] = uint4{0u, 0u, 0u, 0u};
-      if (e < BV) {
-        const int pl = e / (BNC * 2), rem = e - pl * (BNC * 2), q = q0 + (rem >> 1);
-        if (q < ncolc)
-          rb[h] = *reinterpret_cast<const uint4*>(ui + pl * uplane + ((int64_t)ch * ncolc + q) * 16 + 8 * (rem & 1));
+      for (int h = 0; h < BVT; ++h) {
+        const int e = tid + 512 * h;
+        rb[cc][h] = uint4{0u, 0u, 0u, 0u};
+        if (e < BV && ch < nch) {
+          const int pl = e / (BNC * 2), rem = e - pl * (BNC * 2), q = q0 + (rem >> 1);
+          if (q < ncolc)
+            rb[cc][h] =
+                *reinterpret_cast<const uint4*>(ui + pl * uplane + ((int64_t)ch * ncolc + q) * 16 + 8 * (rem & 1));
+        }
       }
     }
   };
   auto stage = [&]() {
 #pragma unroll
-    for (int h = 0; h < AVT; ++h) {
-      const int e = tid + 512 * h;
-      if (e < AV) {
-        const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3;
-        *reinterpret_cast<uint4*>(As[pl] + r * TH_LDK + 8 * v) = ra[h];
-      }
-    }
+    for (int cc = 0; cc < CPS; ++cc) {
 #pragma unroll
-    for (int h = 0; h < BVT; ++h) {
-      const int e = tid + 512 * h;
-      if (e < BV) {
-        // plane pl = 2*comp + (lo); columns (q, re) = [u_re | -u_im], (q, im) = [u_im | u_re]
-        const int pl = e / (BNC * 2), rem = e - pl * (BNC * 2), qc = rem >> 1, half = rem & 1;
-        u16* dst = Bs[pl & 1];
-        const uint4 x = rb[h];
-        if (pl < 2) {
-          *reinterpret_cast<uint4*>(dst + (2 * qc) * TH_LDK + 8 * half) = x;
-          *reinterpret_cast<uint4*>(dst + (2 * qc + 1) * TH_LDK + 16 + 8 * half) = x;
-        } else {
-          const uint4 nx = uint4{x.x ^ 0x80008000u, x.y ^ 0x80008000u, x.z ^ 0x80008000u, x.w ^ 0x80008000u};
-          *reinterpret_cast<uint4*>(dst + (2 * qc + 1) * TH_LDK + 8 * half) = x;
-          *reinterpret_cast<uint4*>(dst + (2 * qc) * TH_LDK + 16 + 8 * half) = nx;
+      for (int h = 0; h < AVT; ++h) {
+        const int e = tid + 512 * h;
+        if (e < AV) {
+          const int pl = e / (BM * 4), r = (e >> 2) - pl * BM, v = e & 3;
+          *reinterpret_cast<uint4*>(As[pl] + r * LDK + 32 * cc + 8 * v) = ra[cc][h];
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < BVT; ++h) {
+        const int e = tid + 512 * h;
+        if (e < BV) {
+          // plane pl = 2*comp + (lo); columns (q, re) = [u_re | -u_im], (q, im) = [u_im | u_re]
+          const int pl = e / (BNC * 2), rem = e - pl * (BNC * 2), qc = rem >> 1, half = rem & 1;
+          u16* dst = Bs[pl & 1] + 32 * cc;
+          const uint4 x = rb[cc][h];
+          if (pl < 2) {
+            *reinterpret_cast<uint4*>(dst + (2 * qc) * LDK + 8 * half) = x;
+            *reinterpret_cast<uint4*>(dst + (2 * qc + 1) * LDK + 16 + 8 * half) = x;
+          } else {
+            const uint4 nx = uint4{x.x ^ 0x80008000u, x.y ^ 0x80008000u, x.z ^ 0x80008000u, x.w ^ 0x80008000u};
+            *reinterpret_cast<uint4*>(dst + (2 * qc + 1) * LDK + 8 * half) = x;
+            *reinterpret_cast<uint4*>(dst + (2 * qc) * LDK + 16 + 8 * half) = nx;
+          }
         }
       }
     }
@@ -483,27 +493,31 @@ __global__ __launch_bounds__(512) void k_taylor_h(const u16* Vh, const u16* Vl, 
   for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int j = 0; j < NI; ++j) c[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nst = (nch + CPS - 1) / CPS;
   fetch(0);
-  for (int ch = 0; ch < nch; ++ch) {
+  for (int st = 0; st < nst; ++st) {
     stage();
     __syncthreads();
-    if (ch + 1 < nch) fetch(ch + 1);
-    const int koff = 8 * (lane >> 4);
-    bf16x8 b[2][NI];
+    if (st + 1 < nst) fetch((st + 1) * CPS);
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int cc = 0; cc < CPS; ++cc) {
+      const int koff = 32 * cc + 8 * (lane >> 4);
+      bf16x8 b[2][NI];
 #pragma unroll
-      for (int j = 0; j < NI; ++j)
-        b[h][j] = *reinterpret_cast<const bf16x8*>(Bs[h] + (wn * 16 * NI + j * 16 + (lane & 15)) * TH_LDK + koff);
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int i = 0; i < MI; ++i) {
-      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As[0] + (wm * 16 * MI + i * 16 + (lane & 15)) * TH_LDK + koff);
-      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As[1] + (wm * 16 * MI + i * 16 + (lane & 15)) * TH_LDK + koff);
+        for (int j = 0; j < NI; ++j)
+          b[h][j] = *reinterpret_cast<const bf16x8*>(Bs[h] + (wn * 16 * NI + j * 16 + (lane & 15)) * LDK + koff);
 #pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[0][j], c[i][j], 0, 0, 0);
-        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[1][j], c[i][j], 0, 0, 0);
-        c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b[0][j], c[i][j], 0, 0, 0);
+      for (int i = 0; i < MI; ++i) {
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(As[0] + (wm * 16 * MI + i * 16 + (lane & 15)) * LDK + koff);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(As[1] + (wm * 16 * MI + i * 16 + (lane & 15)) * LDK + koff);
+#pragma unroll
+        for (int j = 0; j < NI; ++j) {
+          c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[0][j], c[i][j], 0, 0, 0);
+          c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[1][j], c[i][j], 0, 0, 0);
+          c[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b[0][j], c[i][j], 0, 0, 0);
+        }
       }
     }
     __syncthreads();
@@ -770,14 +784,14 @@ void launch_expmul_pre(const DevState& s, hipStream_t st) {
                        n, s.W, ws, 2 * s.det_off[d], 0, gx, gy);
   }
 }
-template <int MI, int NI>
+template <int MI, int NI, int CPS = 1>   // two chunks per stage measured slower at N=300
 static void taylor_split_steps(const DevState& s, hipStream_t st) {
   const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
   const int gx = (int)cdivu(na + nb, 32 * NI), gy = (int)cdivu(s.N, 32 * MI);
   const u16* src = s.Ub[2];
   for (int p = 5; p >= 1; --p) {
     u16* dst = s.Ub[p & 1];
-    hipLaunchKernelGGL((k_taylor_h<MI, NI>), dim3(gx * gy * s.W), dim3(512), 0, st, s.Vh, s.Vl, src, dst, s.Ub[2],
+    hipLaunchKernelGGL((k_taylor_h<MI, NI, CPS>), dim3(gx * gy * s.W), dim3(512), 0, st, s.Vh, s.Vl, src, dst, s.Ub[2],
                        s.t0, s.accb, s.weights, s.N, s.Nq, na, nb, s.det_off[1], 2 * s.ms,
                        (float)(sqrt(s.dt) / p), p == 1 ? 1 : 0, gx, gy, s.W);
     src = dst;
@@ -2115,7 +2129,7 @@ template <int NS>
 __global__ __launch_bounds__(256, 2) void k_vhs_bf16(const u16* Fh, const u16* Fl, const u16* Lh, const u16* Ll,
                                                      double* P, const double* weights, int W, const int2* blk, int N,
                                                      int Gp, int gridx, int gridy, u16* Vh, u16* Vl, int Nq) {
-  using Eng = MmaBF16<128, 128, 2, 2, NS, NS>;
+  using Eng = MmaBF16<128, 128, 2, 2, NS, NS, 64>;
   __shared__ __attribute__((aligned(16))) unsigned char lds[Eng::LDS_BYTES];
   const TileId tl = tile_id(gridx, gridy, 1, 8);
   const int row0 = tl.y * 128;
