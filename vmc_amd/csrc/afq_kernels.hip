@@ -1676,13 +1676,15 @@ void launch_theta_split(const DevState& s, const int* sel, int nsel, hipStream_t
 // Xp as k_exchange, f = R_g theta from bf16 hi/lo operands (NS = 2: BF16x3, NS = 1: BF16),
 // FP32 accumulation, the f_ij f_ji reduction in FP64.
 template <int NS>
-__global__ __launch_bounds__(256, 2) void k_exchange_bf16(const u16* Rh, const u16* Rl, const u16* Th, const u16* Tl,
+__global__ __launch_bounds__(256, NS == 1 ? 4 : 2) void k_exchange_bf16(const u16* Rh, const u16* Rl, const u16* Th,
+                                                                        const u16* Tl,
                                                           const int* sel, double* Xp, int Np, int GE, int nw, int W,
                                                           int G, Slot s0, Slot s1, int ctot, int ngt, int gridx,
                                                           int gridy, int gridz) {
-  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, 64>;
+  // one pass: 32-deep stages keep the kernel within 128 VGPRs, four workgroups per CU
+  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, (NS == 1 ? 32 : 64)>;
   // f of the whole tile at once when it fits in the engine's LDS (NS = 2); else one
-  // g-block at a time, which keeps the one-pass kernel at three workgroups per CU
+  // g-block at a time
   constexpr int LDF = XC_BN + 2;
   constexpr bool ONE_PASS = XC_BM * LDF * 4 <= Eng::LDS_BYTES;
   constexpr int FT_BYTES = (ONE_PASS ? XC_BM : XC_BM / 2) * LDF * 4;
