@@ -102,6 +102,8 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   if ((rc = afq_set_walkers(c, dets.data(), nullptr, nullptr))) return rc;
   dets.clear();
   dets.shrink_to_fit();
+  // Taylor tile shape for this population, timed once before the walk
+  if ((rc = afq_autotune(c, nullptr, nullptr))) return rc;
   std::vector<double> eloc(W), weights(W, 1.0);
   const int nsweeps = p->nsweeps, nsteps = p->nsteps;
   std::vector<double> totalWeights(nsweeps, 0.0), totalEnergies(nsweeps, 0.0);

@@ -874,7 +874,7 @@ constexpr int GN_MAX = 64;
 //     W * 2n real columns (FP64 MFMA), into Sg[w][slot] (64 x 128 doubles each);
 //  2. k_invert: S^-1 and log det S per (walker, slot) by Gauss-Jordan elimination with
 //     partial pivoting, the matrix held in registers (lane = row, wave q = complex
-//     columns 16q..16q+15), so a workgroup needs only 1 KB of LDS;
+//     columns CPW q .. CPW q + CPW - 1), so a workgroup needs only 3 KB of LDS;
 //  3. k_theta: theta = psi S^-1 (FP64 MFMA, S^-1 read from Sg) and the one-body
 //     partial e1 = sum theta o (h1 Phi).
 struct GreenArgs {
@@ -912,7 +912,8 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-__global__ __launch_bounds__(256, 3) void k_invert(GreenArgs a) {
+template <int CPW>   // complex columns per wave; 64 / CPW waves
+__global__ __launch_bounds__(64 * (64 / CPW), CPW == 16 ? 3 : 2) void k_invert(GreenArgs a) {
   __shared__ cplx colb[2][GN_MAX];
   __shared__ cplx pvs[GN_MAX];
   __shared__ int pk[GN_MAX];
@@ -921,21 +922,21 @@ __global__ __launch_bounds__(256, 3) void k_invert(GreenArgs a) {
   const Slot sl = si == 0 ? a.s0 : a.s1;
   const int n = sl.n, tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
   double* S = a.Sg + ((int64_t)w * a.nslots + si) * SG_MAT;
-  cplx x[16];
+  cplx x[CPW];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int col = 16 * q + j;
+  for (int j = 0; j < CPW; ++j) {
+    const int col = CPW * q + j;
     x[j] = cplx{0.0, 0.0};
     if (lane < n && col < n) x[j] = cplx{S[lane * SG_LD + 2 * col], S[lane * SG_LD + 2 * col + 1]};
   }
   bool used = lane >= n;
   int myk = 0;
   for (int k = 0; k < n; ++k) {
-    const int o = k >> 4, c = k & 15;
+    const int o = k / CPW, c = k % CPW;
     if (q == o) {
       cplx sel = x[0];
 #pragma unroll
-      for (int j = 1; j < 16; ++j)
+      for (int j = 1; j < CPW; ++j)
         if (j == c) sel = x[j];
       colb[k & 1][lane] = sel;
     }
@@ -961,7 +962,7 @@ __global__ __launch_bounds__(256, 3) void k_invert(GreenArgs a) {
     // row p / pv, and every other row minus f_i times it; the stored column k turns
     // into column p of the inverse (in place)
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < CPW; ++j) {
       const bool kcol = q == o && j == c;
       cplx r = cmul(cplx{readlane_f64(x[j].re, p), readlane_f64(x[j].im, p)}, pinv);
       if (kcol) r = pinv;
@@ -973,8 +974,8 @@ __global__ __launch_bounds__(256, 3) void k_invert(GreenArgs a) {
   // S^-1[k][p_j] = X[p_k][j]: lane p_k holds row myk, the slot of column j holds column p_j
   if (lane < n) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int col = 16 * q + j;
+    for (int j = 0; j < CPW; ++j) {
+      const int col = CPW * q + j;
       if (col < n) {
         const int lc = pk[col];
         S[myk * SG_LD + 2 * lc] = x[j].re;
@@ -1058,7 +1059,7 @@ void launch_green(const DevState& s, bool all, hipStream_t st) {
     const int gx = (int)cdivu((int64_t)s.W * 2 * s.slot[si].n, 128);
     hipLaunchKernelGGL(k_overlap, dim3(gx), dim3(256), 0, st, a, si, gx);
   }
-  hipLaunchKernelGGL(k_invert, dim3(s.W, s.nslots), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_invert<16>, dim3(s.W, s.nslots), dim3(256), 0, st, a);   // 4 waves: faster than 8 at n = 60
   hipLaunchKernelGGL(k_theta, dim3(s.W, s.nslots), dim3(256), 0, st, a);
 }
 
