@@ -1255,9 +1255,8 @@ __global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double
   __shared__ cplx wt[CG * MC];
   __shared__ cplx tau[CG * MC];
   __shared__ double red[16];
-  __shared__ double sbeta;
-  __shared__ cplx sden, sc0;
-  __shared__ int szero;
+  __shared__ double betas[CG * MC];
+  __shared__ cplx sc0;
   const int tid = threadIdx.x, cg = tid % CG, rg = tid / CG, lane = tid & 63, wave = tid >> 6;
   const int w = blockIdx.x;
   double lf_re = 0.0, lf_im = 0.0;
@@ -1340,29 +1339,24 @@ __global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double
         if (own && i > k && i < N) t += col[m].re * col[m].re + col[m].im * col[m].im;
         if (own && i == k) sc0 = col[m];
       }
+      // every thread forms the reflector's scalars from the tail norm and the pivot
+      // c0 (written before block_sum's barriers); logs of beta wait for the end
       const double tail = block_sum(t, red);
-      if (tid == 0) {
-        const cplx c0 = sc0;
-        if (tail <= DBL_MIN && c0.im * c0.im <= DBL_MIN) {
-          tau[k] = cplx{0.0, 0.0};
-          sbeta = c0.re;
-          szero = 1;
-        } else {
-          double beta = sqrt(c0.re * c0.re + c0.im * c0.im + tail);
-          if (c0.re >= 0.0) beta = -beta;
-          sbeta = beta;
-          sden = cplx{c0.re - beta, c0.im};
-          const cplx tt = cdiv(cplx{beta - c0.re, -c0.im}, cplx{beta, 0.0});
-          tau[k] = cplx{tt.re, -tt.im};
-          szero = 0;
-        }
-        lf_re += log(fabs(sbeta));
-        if (sbeta < 0.0) lf_im += M_PI;
+      const cplx c0 = sc0;
+      const bool zero = tail <= DBL_MIN && c0.im * c0.im <= DBL_MIN;
+      double beta = c0.re;
+      cplx den{1.0, 0.0}, tauk{0.0, 0.0};
+      if (!zero) {
+        beta = sqrt(c0.re * c0.re + c0.im * c0.im + tail);
+        if (c0.re >= 0.0) beta = -beta;
+        den = cplx{c0.re - beta, c0.im};
+        const cplx tt = cdiv(cplx{beta - c0.re, -c0.im}, cplx{beta, 0.0});
+        tauk = cplx{tt.re, -tt.im};
       }
-      __syncthreads();
-      const double beta = sbeta;
-      const int zero = szero;
-      const cplx den = sden;
+      if (tid == 0) {
+        tau[k] = tauk;
+        betas[k] = beta;
+      }
       if (own) {
 #pragma unroll
         for (int m = 0; m < MR; ++m) {
@@ -1380,8 +1374,15 @@ __global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double
         }
       }
       __syncthreads();
-      reduce_w(k, k + 1, tau[k]);
+      reduce_w(k, k + 1, tauk);
       update(k, k + 1);
+    }
+    // log prod beta over this determinant's reflectors (betas settled by the barriers above)
+    if (wave == 0) {
+      const double lb = lane < n ? log(fabs(betas[lane])) : 0.0;
+      const double ng = (lane < n && betas[lane] < 0.0) ? 1.0 : 0.0;
+      lf_re += wave_sum(lb);
+      lf_im += M_PI * wave_sum(ng);
     }
     // park the reflectors, then Q = H_0^* ... H_{n-1}^* [I; 0],  H_k^* = I - conj(tau_k) v v^*
 #pragma unroll
@@ -1393,10 +1394,15 @@ __global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double
         a[m][c] = cplx{i == j ? 1.0 : 0.0, 0.0};
       }
     __syncthreads();
+    // reflector columns: this thread's first row of column k - 1 is in flight while
+    // column k is applied
+    auto vload = [&](int k, int i) -> cplx { return i > k ? V[(int64_t)i * n + k] : cplx{i == k ? 1.0 : 0.0, 0.0}; };
+    cplx pre = tid < N ? vload(n - 1, tid) : cplx{0.0, 0.0};
     for (int k = n - 1; k >= 0; --k) {
-      for (int i = tid; i < N; i += QR_T)
-        vl[i] = i > k ? V[(int64_t)i * n + k] : cplx{i == k ? 1.0 : 0.0, 0.0};
+      if (tid < N) vl[tid] = pre;
+      for (int i = tid + QR_T; i < N; i += QR_T) vl[i] = vload(k, i);
       __syncthreads();
+      if (k > 0 && tid < N) pre = vload(k - 1, tid);
       reduce_w(k, k, cplx{tau[k].re, -tau[k].im});
       update(k, k);
     }
