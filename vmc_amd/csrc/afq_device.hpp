@@ -247,7 +247,9 @@ __device__ __forceinline__ void split_bf16(double x, u16& hi, u16& lo) {
   lo = f2bf_bits((float)(x - (double)bf_bits2f(hi)));
 }
 
-template <int BM, int BN, int WAVES_M, int WAVES_N, int NA, int NB, int BK_ = 32>
+// DB: two LDS stage buffers (the next stage is written while the current one is read),
+// one barrier per stage instead of two.
+template <int BM, int BN, int WAVES_M, int WAVES_N, int NA, int NB, int BK_ = 32, bool DB = false>
 struct MmaBF16 {
   static constexpr int BK = BK_;
   static constexpr int CPR = BK / 8;  // 16-byte chunks per row and stage
@@ -259,7 +261,8 @@ struct MmaBF16 {
   static constexpr int B_CH = NB * BN * CPR;
   static constexpr int A_PER_T = (A_CH + NT - 1) / NT;
   static constexpr int B_PER_T = (B_CH + NT - 1) / NT;
-  static constexpr int LDS_BYTES = (NA * BM + NB * BN) * LDK * 2;
+  static constexpr int STAGE_U16 = (NA * BM + NB * BN) * LDK;
+  static constexpr int LDS_BYTES = STAGE_U16 * 2 * (DB ? 2 : 1);
   static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be a multiple of 16");
   static_assert(BK % 32 == 0, "stage depth must be a multiple of the MFMA K (32)");
 
@@ -332,15 +335,28 @@ struct MmaBF16 {
     }                                                                                          \
   }
     const int nst = (Kp + BK - 1) / BK;
-    if (nst > 0) AFQ_BF16_FETCH(0)
-    for (int s = 0; s < nst; ++s) {
+    auto put = [&](int buf) {
+      u16* base = lds + buf * STAGE_U16;
 #pragma unroll
       for (int q = 0; q < A_PER_T; ++q)
-        if (A_CH % NT == 0 || tid + q * NT < A_CH) *reinterpret_cast<uint4*>(As + oA[q]) = ra[q];
+        if (A_CH % NT == 0 || tid + q * NT < A_CH) *reinterpret_cast<uint4*>(base + oA[q]) = ra[q];
 #pragma unroll
       for (int q = 0; q < B_PER_T; ++q)
-        if (B_CH % NT == 0 || tid + q * NT < B_CH) *reinterpret_cast<uint4*>(Bs + oB[q]) = rb[q];
+        if (B_CH % NT == 0 || tid + q * NT < B_CH) *reinterpret_cast<uint4*>(base + NA * BM * LDK + oB[q]) = rb[q];
+    };
+    if (nst > 0) AFQ_BF16_FETCH(0)
+    if (DB && nst > 0) {
+      put(0);
       __syncthreads();
+    }
+    for (int s = 0; s < nst; ++s) {
+      if (DB) {
+        As = lds + (s & 1) * STAGE_U16;
+        Bs = As + NA * BM * LDK;
+      } else {
+        put(0);
+        __syncthreads();
+      }
       if (s + 1 < nst) AFQ_BF16_FETCH((s + 1) * BK)
 #pragma unroll
       for (int ks = 0; ks < BK / 32; ++ks) {
@@ -365,6 +381,7 @@ struct MmaBF16 {
             if (NA > 1) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[NA - 1][i], b[0][j], acc[i][j], 0, 0, 0);
           }
       }
+      if (DB && s + 1 < nst) put((s + 1) & 1);
       __syncthreads();
     }
 #undef AFQ_BF16_FETCH

@@ -1688,16 +1688,17 @@ __global__ __launch_bounds__(256, NS == 1 ? 4 : 2) void k_exchange_bf16(const u1
                                                           const int* sel, double* Xp, int Np, int GE, int nw, int W,
                                                           int G, Slot s0, Slot s1, int ctot, int ngt, int gridx,
                                                           int gridy, int gridz) {
-  // one pass: 32-deep stages keep the kernel within 128 VGPRs, four workgroups per CU
-  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, (NS == 1 ? 32 : 64)>;
+  // one pass: 32-deep double-buffered stages (one barrier per stage) within 128 VGPRs
+  // and 40 KB of LDS, four workgroups per CU
+  using Eng = MmaBF16<XC_BM, XC_BN, 2, 2, NS, NS, (NS == 1 ? 32 : 64), NS == 1>;
   // f of the whole tile at once when it fits in the engine's LDS (NS = 2); else one
-  // g-block at a time
+  // g-block at a time; the group sums' scratch follows f
   constexpr int LDF = XC_BN + 2;
-  constexpr bool ONE_PASS = XC_BM * LDF * 4 <= Eng::LDS_BYTES;
+  constexpr bool ONE_PASS = XC_BM * LDF * 4 + 64 <= Eng::LDS_BYTES;
   constexpr int FT_BYTES = (ONE_PASS ? XC_BM : XC_BM / 2) * LDF * 4;
-  constexpr int LDSB = Eng::LDS_BYTES > FT_BYTES ? Eng::LDS_BYTES : FT_BYTES;
+  constexpr int LDSB = Eng::LDS_BYTES > FT_BYTES + 64 ? Eng::LDS_BYTES : FT_BYTES + 64;
   __shared__ __attribute__((aligned(16))) unsigned char lds[LDSB];
-  __shared__ double red[8];
+  double* red = reinterpret_cast<double*>(lds + FT_BYTES);
   const TileId tl = tile_id(gridx, gridy, gridz, 8);
   const int zs = gridz > 1 ? tl.z : 0;
   const Slot sl = zs == 0 ? s0 : s1;
