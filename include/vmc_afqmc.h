@@ -184,7 +184,8 @@ int afq_ham_and_overlap(afq_ctx* ctx, const double* dets, int32_t ndet,
                         int32_t nchol_ene, double* out);
 
 /* Parity hooks: force bias (nwalk x nchol complex) and trial overlaps of the
- * current population, computed from the current determinants. */
+ * current population, computed from the current determinants.  The force bias is
+ * kept for walkers of nonzero weight only; a dead walker's entries are stale. */
 int afq_force_bias(afq_ctx* ctx, double* fb);
 int afq_overlap(afq_ctx* ctx, double* overlaps);
 

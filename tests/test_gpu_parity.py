@@ -292,7 +292,8 @@ def test_largest_shape_parity():
     e, ho = ctx.measure()
     _, _, wo = orc.get_walkers()
     for w in range(W):
-        assert rel(fbg[w], orc.force_bias(w)) < TOL, ("fb", w)
+        if wo[w] != 0:   # a dead walker carries no force bias (never propagated or measured)
+            assert rel(fbg[w], orc.force_bias(w)) < TOL, ("fb", w)
         oo = orc.ham_and_overlap(w)
         if wo[w] != 0:
             assert rel_each(ho[w], oo) < TOL, ("hamAndOverlap", w)

@@ -178,6 +178,8 @@ void free_all(DevState& s) {
   for (double* p : ptrs)
     if (p) (void)hipFree(p);
   if (s.idx) (void)hipFree(s.idx);
+  if (s.live) (void)hipFree(s.live);
+  if (s.nlive) (void)hipFree(s.nlive);
   if (s.vhs_blk) (void)hipFree(s.vhs_blk);
   if (s.ncap) (void)hipFree(s.ncap);
   unsigned short* hp[] = {s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo, s.Lt_hi, s.Lt_lo, s.Fb_hi, s.Fb_lo,
@@ -321,6 +323,8 @@ void alloc_walkers(DevState& s) {
   s.ho = dalloc<double>(4 * W);
   s.cum = dalloc<double>(W);
   s.idx = dalloc<int>(W);
+  s.live = dalloc<int>(W);
+  s.nlive = dalloc<int>(1);
   s.ncap = dalloc<unsigned long long>(1);
 }
 

@@ -74,6 +74,8 @@ struct DevState {
   double* ho = nullptr;      // W x 4 {E*O, O}
   double* cum = nullptr;     // W (comb)
   int* idx = nullptr;        // W (comb)
+  int* live = nullptr;       // W: walkers of nonzero weight in ascending order, count in *nlive
+  int* nlive = nullptr;
   unsigned long long* ncap = nullptr;
 
   // mixed precision

@@ -25,12 +25,12 @@ namespace afq {
 static inline unsigned cdivu(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ force bias
-struct FbA {  // A((w,c), k) = theta[w][k][c]
-  const double* th; int64_t Ktot; int row0, nrow;
+struct FbA {  // A((l,c), k) = theta[w][k][c] for the l-th live walker w = live[l]
+  const double* th; const int* live; int64_t Ktot; int row0, nrow;
   __device__ double operator()(int r, int k) const {
     const int q = row0 + r;
     if (q >= nrow || k >= Ktot) return 0.0;
-    return th[(int64_t)(q >> 1) * 2 * Ktot + 2 * (int64_t)k + (q & 1)];
+    return th[(int64_t)live[q >> 1] * 2 * Ktot + 2 * (int64_t)k + (q & 1)];
   }
 };
 struct FbB {  // B(k, g) = R[g][k]
@@ -43,24 +43,59 @@ struct FbB {  // B(k, g) = R[g][k]
 };
 
 constexpr int FB_BM = 128, FB_BN = 128;
+// Force bias of the walkers with nonzero weight only (live[0 .. *nlive)): a walker of
+// weight zero is neither propagated nor measured (DQMC/MixedEstimator.cpp:461-468,531)
+// and the comb replaces it, so its force bias is never read.  (A walker below the
+// propagation threshold 1e-8 but nonzero is still measured, and E_J needs its force
+// bias.)  The grid covers every walker; tiles past the live count exit.
 __global__ __launch_bounds__(256, 2) void k_force_bias(const double* R, const double* th, double* fb,
-                                                    int G, int W, int64_t Ktot, double alpha, int gridx,
-                                                    int gridy) {
+                                                    const int* live, const int* nlive, int G, int64_t Ktot,
+                                                    double alpha, int gridx, int gridy) {
   using Eng = MmaF64<FB_BM, FB_BN, 2, 2, false, false>;
   __shared__ double lds[Eng::LDS_DOUBLES];
   const TileId tl = tile_id(gridx, gridy, 1, 8);
   const int row0 = tl.y * FB_BM, g0 = tl.x * FB_BN;
+  const int nrow = 2 * *nlive;
+  if (row0 >= nrow) return;
   Eng e;
-  e.run(FbA{th, Ktot, row0, 2 * W}, FbB{R, Ktot, g0, G}, (int)Ktot, lds);
+  e.run(FbA{th, live, Ktot, row0, nrow}, FbB{R, Ktot, g0, G}, (int)Ktot, lds);
   e.store([&](int r, int c, double v) {
     const int q = row0 + r, g = g0 + c;
-    if (q < 2 * W && g < G) fb[(int64_t)(q >> 1) * 2 * G + 2 * g + (q & 1)] = alpha * v;
+    if (q < nrow && g < G) fb[(int64_t)live[q >> 1] * 2 * G + 2 * g + (q & 1)] = alpha * v;
   });
 }
 
+// live[0 .. *count) = the walkers of nonzero weight, ascending (one workgroup)
+__global__ __launch_bounds__(1024) void k_live_list(const double* weights, int W, int* live, int* count) {
+  __shared__ int wtot[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int offset = 0;
+  for (int c0 = 0; c0 < W; c0 += 1024) {
+    const int w = c0 + (int)threadIdx.x;
+    const bool on = w < W && weights[w] != 0.0;
+    const unsigned long long m = __ballot(on);
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wtot[wave] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      if (q < wave) base += wtot[q];
+      tot += wtot[q];
+    }
+    if (on) live[offset + base + pre] = w;
+    __syncthreads();
+    offset += tot;
+  }
+  if (threadIdx.x == 0) *count = offset;
+}
+void launch_live_list(const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_live_list, dim3(1), dim3(1024), 0, st, s.weights, s.W, s.live, s.nlive);
+}
+
 void launch_force_bias(const DevState& s, hipStream_t st) {
+  launch_live_list(s, st);
   const int gx = (int)cdivu(s.G, FB_BN), gy = (int)cdivu(2 * (int64_t)s.W, FB_BM);
-  hipLaunchKernelGGL(k_force_bias, dim3(gx * gy), dim3(256), 0, st, s.R, s.theta, s.fb, s.G, s.W, s.Ktot,
+  hipLaunchKernelGGL(k_force_bias, dim3(gx * gy), dim3(256), 0, st, s.R, s.theta, s.fb, s.live, s.nlive, s.G, s.Ktot,
                      (double)s.slot[0].mult, gx, gy);
 }
 
@@ -1648,9 +1683,11 @@ void launch_rsplit(const DevState& s, hipStream_t st) {
 // into Tb[w][re/im][slot][j][k]: every (walker, component) row holds the slots'
 // columns back to back, so the force bias reads it as one K-contiguous row and the
 // exchange reads each (walker, component, slot, j) as a K-contiguous column
-__global__ __launch_bounds__(256) void k_theta_split(const double* theta, const int* sel, u16* hi, u16* lo, int N,
+__global__ __launch_bounds__(256) void k_theta_split(const double* theta, const int* sel, const int* nsel_dev,
+                                                     u16* hi, u16* lo, int N,
                                                      int Np, int64_t Ktot, Slot s0, Slot s1, int ctot) {
   __shared__ double tile[32 * 129];
+  if (nsel_dev && (int)blockIdx.x >= *nsel_dev) return;
   const int w = sel ? sel[blockIdx.x] : blockIdx.x;
   const Slot sl = blockIdx.y == 0 ? s0 : s1;
   const int n2 = 2 * sl.n;
@@ -1676,7 +1713,7 @@ __global__ __launch_bounds__(256) void k_theta_split(const double* theta, const 
   }
 }
 void launch_theta_split(const DevState& s, const int* sel, int nsel, hipStream_t st) {
-  hipLaunchKernelGGL(k_theta_split, dim3(nsel, s.nslots), dim3(256), 0, st, s.theta, sel, s.Tb_hi, s.Tb_lo, s.N,
+  hipLaunchKernelGGL(k_theta_split, dim3(nsel, s.nslots), dim3(256), 0, st, s.theta, sel, nullptr, s.Tb_hi, s.Tb_lo, s.N,
                      s.Np, s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ctot);
 }
 
@@ -2037,13 +2074,15 @@ static void launch_exchange_q8(const DevState& s, const int* sel, int nsel, hipS
 // n_s * Np per slot, FP32 accumulation.
 template <int NS>
 __global__ __launch_bounds__(256, 2) void k_force_bias_bf16(const u16* Rh, const u16* Rl, const u16* Th,
-                                                            const u16* Tl, double* fb, int G, int W, int Np,
-                                                            int nsum, int nslots, Slot s0, Slot s1, double alpha,
-                                                            int gridx, int gridy) {
+                                                            const u16* Tl, double* fb, const int* live,
+                                                            const int* nlive, int G, int Np, int nsum, int nslots,
+                                                            Slot s0, Slot s1, double alpha, int gridx, int gridy) {
   using Eng = MmaBF16<128, 128, 2, 2, NS, NS>;
   __shared__ __attribute__((aligned(16))) unsigned char lds[Eng::LDS_BYTES];
   const TileId tl = tile_id(gridx, gridy, 1, 8);
   const int row0 = tl.y * 128, g0 = tl.x * 128;
+  const int nrow = 2 * *nlive;   // live walkers only, as k_force_bias
+  if (row0 >= nrow) return;
   Eng e;
   int64_t roff = 0;
   int coff = 0;
@@ -2054,8 +2093,8 @@ __global__ __launch_bounds__(256, 2) void k_force_bias_bf16(const u16* Rh, const
     e.run(
         [&](int c, int r) -> const u16* {
           const int q = row0 + r;
-          if (q >= 2 * W) return nullptr;
-          return (c ? Tl : Th) + ((int64_t)q * nsum + co) * Np;
+          if (q >= nrow) return nullptr;
+          return (c ? Tl : Th) + ((int64_t)(2 * live[q >> 1] + (q & 1)) * nsum + co) * Np;
         },
         [&](int c, int col) -> const u16* {
           const int g = g0 + col;
@@ -2069,20 +2108,24 @@ __global__ __launch_bounds__(256, 2) void k_force_bias_bf16(const u16* Rh, const
   }
   e.store([&](int r, int c, float v) {
     const int q = row0 + r, g = g0 + c;
-    if (q < 2 * W && g < G) fb[(int64_t)(q >> 1) * 2 * G + 2 * g + (q & 1)] = alpha * (double)v;
+    if (q < nrow && g < G) fb[(int64_t)live[q >> 1] * 2 * G + 2 * g + (q & 1)] = alpha * (double)v;
   });
 }
 
 void launch_force_bias_lp(const DevState& s, int prec, hipStream_t st) {
-  launch_theta_split(s, nullptr, s.W, st);
+  launch_live_list(s, st);
+  hipLaunchKernelGGL(k_theta_split, dim3(s.W, s.nslots), dim3(256), 0, st, s.theta, s.live, s.nlive, s.Tb_hi,
+                     s.Tb_lo, s.N, s.Np, s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ctot);
   const int gx = (int)cdivu(s.G, 128), gy = (int)cdivu(2 * (int64_t)s.W, 128);
   const Slot& b = s.slot[s.nslots > 1 ? 1 : 0];
   if (prec == AFQ_PREC_BF16X3)
     hipLaunchKernelGGL(k_force_bias_bf16<2>, dim3(gx * gy), dim3(256), 0, st, s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo,
-                       s.fb, s.G, s.W, s.Np, s.ctot / 2, s.nslots, s.slot[0], b, (double)s.slot[0].mult, gx, gy);
+                       s.fb, s.live, s.nlive, s.G, s.Np, s.ctot / 2, s.nslots, s.slot[0], b, (double)s.slot[0].mult,
+                       gx, gy);
   else
     hipLaunchKernelGGL(k_force_bias_bf16<1>, dim3(gx * gy), dim3(256), 0, st, s.Rb_hi, s.Rb_lo, s.Tb_hi, s.Tb_lo,
-                       s.fb, s.G, s.W, s.Np, s.ctot / 2, s.nslots, s.slot[0], b, (double)s.slot[0].mult, gx, gy);
+                       s.fb, s.live, s.nlive, s.G, s.Np, s.ctot / 2, s.nslots, s.slot[0], b, (double)s.slot[0].mult,
+                       gx, gy);
 }
 
 // ------------------------------------------------------------------ mixed-precision VHS
