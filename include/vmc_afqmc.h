@@ -163,8 +163,10 @@ int afq_set_weights(afq_ctx* ctx, const double* weights);
 int afq_propagate(afq_ctx* ctx, const double* draws, uint64_t seed, uint64_t step,
                   double eshift, double* factors, int64_t* n_capped);
 
-/* Householder re-orthonormalisation of every walker (Eigen's sign convention,
- * R_kk = -sign(Re x_0) |x|), trialOverlap /= prod diag R. */
+/* Householder re-orthonormalisation of every walker of nonzero weight (Eigen's sign
+ * convention, R_kk = -sign(Re x_0) |x|), trialOverlap /= prod diag R.  A walker of
+ * weight zero is left as it is: it is neither propagated nor measured before the comb
+ * replaces it. */
 int afq_orthogonalize(afq_ctx* ctx);
 
 /* Local energies of every walker with weight != 0; dead walkers get eloc 0 and

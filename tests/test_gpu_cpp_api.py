@@ -70,13 +70,14 @@ def test_cpp_api_matches_oracle(name, tmp_path):
     elg = take(np.float64, W)
     assert off == buf.size
     do, oo, wo = orc.get_walkers()
-    assert rel(dg, do) < TOL and rel(wg, wo) < TOL
     live = wo > 0
     assert live.any()
+    # a walker of weight zero is skipped by the re-orthonormalisation and the force bias
+    assert rel(dg[live], do[live]) < TOL and rel(wg, wo) < TOL
     assert rel_each(og[live], oo[live]) < TOL
     for w in range(W):
-        assert rel(fbg[w], orc.force_bias(w)) < TOL
         if live[w]:
+            assert rel(fbg[w], orc.force_bias(w)) < TOL
             ho = orc.ham_and_overlap(w)
             assert rel_each(hog[w], ho) < TOL
             assert abs(elg[w] - (ho[0] / ho[1]).real) < TOL * abs(elg[w])

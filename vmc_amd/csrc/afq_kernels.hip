@@ -1159,8 +1159,8 @@ void launch_logO_from_logdet(const DevState& s, hipStream_t st) {
 // Thin Householder QR with Eigen's conventions (eigen/Eigen/src/Householder/Householder.h:63-92,
 // eigen/Eigen/src/QR/HouseholderQR.h:267-297); psi_d := Q, logO -= log prod diag R
 // (DQMC/DQMCWalker.cpp:295-326).  One workgroup per walker, both determinants.
-__global__ __launch_bounds__(256) void k_qr(double* psi, double* Qs, double* logO, int N, int ndets,
-                                            int na, int nb, int64_t offb, int64_t ms, int rhf_walker) {
+__global__ __launch_bounds__(256) void k_qr(double* psi, double* Qs, double* logO, const double* weights, int N,
+                                            int ndets, int na, int nb, int64_t offb, int64_t ms, int rhf_walker) {
   __shared__ cplx tau[GN_MAX];
   __shared__ cplx tmp[GN_MAX];
   __shared__ double red[16];
@@ -1168,6 +1168,7 @@ __global__ __launch_bounds__(256) void k_qr(double* psi, double* Qs, double* log
   __shared__ cplx sden;
   __shared__ int szero;
   const int w = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwv = blockDim.x >> 6;
+  if (weights[w] == 0.0) return;   // weight zero: never read again before the comb replaces it
   double lf_re = 0.0, lf_im = 0.0;
   for (int d = 0; d < ndets; ++d) {
     const int n = d == 0 ? na : nb;
@@ -1282,8 +1283,8 @@ __global__ __launch_bounds__(256) void k_qr(double* psi, double* Qs, double* log
 // (LDS), and the rank-1 update is local.  The reflectors are parked in Vs (the walker's
 // scratch block) and applied backwards to [I; 0] the same way to form the thin Q.
 template <int QR_T, int CG, int MR, int MC>
-__global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double* logO, int N, int ndets, int na,
-                                                 int nb, int64_t offb, int64_t ms, int rhf_walker) {
+__global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double* logO, const double* weights, int N,
+                                                 int ndets, int na, int nb, int64_t offb, int64_t ms, int rhf_walker) {
   constexpr int RG = QR_T / CG, QR_WAVES = QR_T / 64;
   __shared__ cplx vl[RG * MR];
   __shared__ cplx wsum[QR_WAVES][CG * MC];
@@ -1294,6 +1295,7 @@ __global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double
   __shared__ cplx sc0;
   const int tid = threadIdx.x, cg = tid % CG, rg = tid / CG, lane = tid & 63, wave = tid >> 6;
   const int w = blockIdx.x;
+  if (weights[w] == 0.0) return;   // weight zero: never read again before the comb replaces it
   double lf_re = 0.0, lf_im = 0.0;
   for (int d = 0; d < ndets; ++d) {
     const int n = d == 0 ? na : nb;
@@ -1459,7 +1461,8 @@ __global__ __launch_bounds__(QR_T) void k_qr_reg(double* psi, double* Vs, double
 
 template <int T, int CG, int MR, int MC>
 static void launch_qr_reg(const DevState& s, hipStream_t st) {
-  hipLaunchKernelGGL((k_qr_reg<T, CG, MR, MC>), dim3(s.W), dim3(T), 0, st, s.psi, s.t0, s.logO, s.N, s.ndets_prop,
+  hipLaunchKernelGGL((k_qr_reg<T, CG, MR, MC>), dim3(s.W), dim3(T), 0, st, s.psi, s.t0, s.logO, s.weights, s.N,
+                     s.ndets_prop,
                      s.det_n[0], s.det_n[1], s.det_off[1], s.ms, (int)s.walker_rhf);
 }
 
@@ -1480,7 +1483,7 @@ void launch_qr(const DevState& s, hipStream_t st) {
     else launch_qr_reg<512, 8, 5, 8>(s, st);
     return;
   }
-  hipLaunchKernelGGL(k_qr, dim3(s.W), dim3(256), 0, st, s.psi, s.t0, s.logO, s.N, s.ndets_prop,
+  hipLaunchKernelGGL(k_qr, dim3(s.W), dim3(256), 0, st, s.psi, s.t0, s.logO, s.weights, s.N, s.ndets_prop,
                      s.det_n[0], s.det_n[1], s.det_off[1], s.ms, (int)s.walker_rhf);
 }
 
