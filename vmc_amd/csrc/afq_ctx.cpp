@@ -762,6 +762,7 @@ int afq_set_weights(afq_ctx* c, const double* weights) {
   if (!c || !weights) return AFQ_ERR_ARG;
   AFQ_GUARD_BEGIN
   upload(c->s.weights, weights, c->s.W, c->stream);
+  c->s.tb_live = false;
   hip_check(hipStreamSynchronize(c->stream), "sync");
   return AFQ_OK;
   AFQ_GUARD_END(c)

@@ -85,6 +85,10 @@ struct DevState {
   unsigned short* Rb_hi = nullptr; unsigned short* Rb_lo = nullptr;  // [slot][g][i][Np]
   unsigned short* Tb_hi = nullptr; unsigned short* Tb_lo = nullptr;  // [w][re/im][slot][j][Np]
   int prec_fb = AFQ_PREC_FP64, prec_vhs = AFQ_PREC_FP64;
+  // Tb holds the split theta of every walker of nonzero weight (set by the split-BF16
+  // force bias; cleared by whatever changes theta or the weights), so the exchange of
+  // the same walkers need not split again
+  mutable bool tb_live = false;
   int prec_prop = AFQ_PREC_FP64;   // Taylor products V t
   // split-BF16 Taylor operands (allocated when prec_prop is BF16X3); orbitals come in
   // chunks of 16 (Nq = N padded to 16, pads stay zero):

@@ -1085,6 +1085,7 @@ __global__ __launch_bounds__(256, 2) void k_theta(GreenArgs a) {
 }
 
 void launch_green(const DevState& s, bool all, hipStream_t st) {
+  s.tb_live = false;
   GreenArgs a;
   a.psi = s.psi; a.theta = s.theta; a.logdet = s.logdet; a.e1 = s.e1; a.weights = s.weights; a.Sg = s.Sg;
   a.phi0 = s.phi[0]; a.phi1 = s.phi[1]; a.hphi0 = s.hphi[0]; a.hphi1 = s.hphi[1];
@@ -1797,7 +1798,7 @@ void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, h
     launch_exchange_q8(s, sel, nsel, st);
     return;
   }
-  launch_theta_split(s, sel, nsel, st);
+  if (!s.tb_live) launch_theta_split(s, sel, nsel, st);   // else split by the force bias already
   auto go = [&](int z0, int nz, double* xp) {
     const Slot& a = s.slot[z0];
     const int gt = exchange_tile_g(a.n), wt = exchange_tile_w(a.n);
@@ -2119,6 +2120,7 @@ void launch_force_bias_lp(const DevState& s, int prec, hipStream_t st) {
   launch_live_list(s, st);
   hipLaunchKernelGGL(k_theta_split, dim3(s.W, s.nslots), dim3(256), 0, st, s.theta, s.live, s.nlive, s.Tb_hi,
                      s.Tb_lo, s.N, s.Np, s.Ktot, s.slot[0], s.slot[s.nslots > 1 ? 1 : 0], s.ctot);
+  s.tb_live = true;
   const int gx = (int)cdivu(s.G, 128), gy = (int)cdivu(2 * (int64_t)s.W, 128);
   const Slot& b = s.slot[s.nslots > 1 ? 1 : 0];
   if (prec == AFQ_PREC_BF16X3)
@@ -2365,6 +2367,7 @@ void launch_step_inc(unsigned long long* step, hipStream_t st) {
 }
 
 void launch_comb(DevState& s, double zeta, hipStream_t st) {
+  s.tb_live = false;
   hipLaunchKernelGGL(k_cumsum, dim3(1), dim3(64), 0, st, s.weights, s.cum, s.W);
   hipLaunchKernelGGL(k_comb_index, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.cum, s.idx, s.weights, s.W, zeta,
                      s.d_zeta);
