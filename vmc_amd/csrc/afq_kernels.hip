@@ -1740,7 +1740,7 @@ __global__ __launch_bounds__(256, NS == 1 ? 4 : 2) void k_exchange_bf16(const u1
   constexpr int LDSB = Eng::LDS_BYTES > FT_BYTES + 64 ? Eng::LDS_BYTES : FT_BYTES + 64;
   __shared__ __attribute__((aligned(16))) unsigned char lds[LDSB];
   double* red = reinterpret_cast<double*>(lds + FT_BYTES);
-  const TileId tl = tile_id(gridx, gridy, gridz, 8);
+  const TileId tl = tile_id(gridx, gridy, gridz, NS == 1 ? 32 : 8);   // walkers per L2 group
   const int zs = gridz > 1 ? tl.z : 0;
   const Slot sl = zs == 0 ? s0 : s1;
   const int n = sl.n, gt = XC_BM / n, wt = XC_BN / (2 * n);
