@@ -93,7 +93,8 @@ def run_path(ham, spec, total_steps, warmup, **kw):
                 cv_fallbacks=int(res.cv_fallbacks), train_samples=int(res.train_samples),
                 shadow_n=int(res.shadow_n), shadow_shift=float(res.shadow_shift),
                 shadow_var=float(res.shadow_var), shadow_est_var=float(res.shadow_est_var),
-                shadow_wall_s=float(res.shadow_wall_s), live_frac=float(res.live_frac))
+                shadow_wall_s=float(res.shadow_wall_s), live_frac=float(res.live_frac),
+                shadow_clip_frac=float(res.shadow_clip_frac))
 
 
 HBM_PEAK_GBS = 8000.0
@@ -204,9 +205,9 @@ def profiled_traffic(workload, kernel):
         return None
     for line in open(PROFILE_HBM):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and cells[0] == kernel:
+        if len(cells) == 7 and cells[0] == kernel:
             try:
-                return (float(cells[3]) + float(cells[5])) * 2 ** 20
+                return float(cells[6]) * 2 ** 20   # fetch + write over the timed steps' dispatches
             except ValueError:
                 return None
     return None
@@ -231,6 +232,7 @@ def cv_summary(mx, fp64):
                 energy_shift_sem_mha=None if sem is None else 1e3 * sem,
                 energy_shift_blocks=n,
                 cv_noise_per_block_mha=1e3 * float(np.sqrt(mx["shadow_var"])),
+                clipped_fraction=mx.get("shadow_clip_frac"),
                 fp64_error_per_block_mha=1e3 * float(np.sqrt(mx["shadow_est_var"])),
                 speedup_vs_fp64=mx["walker_steps_per_s"] / fp64["walker_steps_per_s"],
                 err_per_sqrt_s_ratio=ratio)

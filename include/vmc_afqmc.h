@@ -360,6 +360,8 @@ typedef struct afq_run_result {
   double shadow_est_var;       /* est_var of the exact FP64 energies of the same blocks */
   double shadow_wall_s;        /* wall seconds spent in shadow measurements (not timed) */
   double live_frac;            /* mean fraction of live walkers (weight > 0) over the timed steps */
+  double shadow_clip_frac;     /* mean fraction of the live walkers whose exact E_L lies outside the
+                                  clip window (running average +- sqrt(2/dt)), over the shadow blocks */
 } afq_run_result;
 
 /* ref_up/ref_dn: reference determinant (rhf.txt or uhf.txt by walker type,

@@ -429,7 +429,7 @@ class RunResult(C.Structure):
                 ("cv_var_sub", C.c_double), ("cv_rms", C.c_double), ("cv_fallbacks", C.c_int64),
                 ("train_samples", C.c_int64), ("est_var", C.c_double), ("shadow_n", C.c_int64),
                 ("shadow_shift", C.c_double), ("shadow_var", C.c_double), ("shadow_est_var", C.c_double),
-                ("shadow_wall_s", C.c_double), ("live_frac", C.c_double)]
+                ("shadow_wall_s", C.c_double), ("live_frac", C.c_double), ("shadow_clip_frac", C.c_double)]
 
 
 def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, nsweeps, ortho_steps,

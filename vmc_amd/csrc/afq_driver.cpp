@@ -149,7 +149,7 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   int64_t est_n = 0;
   // shadow exact measurements of the surrogate blocks
   std::vector<double> eloc_sh(W), centers(W);
-  double sh_sum = 0.0, sh_sum2 = 0.0, sh_est = 0.0, sh_wall = 0.0;
+  double sh_sum = 0.0, sh_sum2 = 0.0, sh_est = 0.0, sh_wall = 0.0, sh_clip = 0.0;
   double sh_stage[AFQ_NSTAGES] = {0};
   int64_t sh_n = 0;
   double live_sum = 0.0;
@@ -247,6 +247,15 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
         afq_stage_times(c, after, 0);
         if (step > p->warmup_steps)
           for (int i = 0; i < AFQ_NSTAGES; ++i) sh_stage[i] += after[i] - before[i];
+        if (step > p->warmup_steps) {   // how much of the block the clip decides
+          int nl = 0, nc = 0;
+          for (int w = 0; w < W; ++w)
+            if (weights[w] != 0.0) {
+              ++nl;
+              nc += std::fabs(eloc_sh[w] - averageEnergy[w / nw]) > clip_w;
+            }
+          sh_clip += nl ? (double)nc / nl : 0.0;
+        }
         if (!clip_host(eloc_sh, false)) return rc = AFQ_ERR_NUMERIC;
         std::vector<double> ecv = eloc;
         if (!clipped_on_device && !clip_host(ecv, false)) return rc = AFQ_ERR_NUMERIC;
@@ -344,6 +353,7 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     res->shadow_var = sh_n > 1 ? (sh_sum2 - sh_sum * sh_sum / sh_n) / (sh_n - 1) : 0.0;
     res->shadow_est_var = sh_n ? sh_est / sh_n : 0.0;
     res->shadow_wall_s = sh_wall;
+    res->shadow_clip_frac = sh_n ? sh_clip / sh_n : 0.0;
     res->live_frac = timed ? live_sum / timed : 0.0;
   }
   return AFQ_OK;
