@@ -373,6 +373,10 @@ def main():
                          "always measured exactly")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned-surrogate path")
+    ap.add_argument("--mlp", action="store_true",
+                    help="run the learned-surrogate path at N=300 too (by default only at the smaller "
+                         "workloads: it first collects exact pairs for several steps, and by then the N=300 "
+                         "population has collapsed below the propagation threshold)")
     ap.add_argument("--surrogate-prec", choices=["bf16", "bf16x3", "fp8"], default="bf16",
                     help="precision of the exchange surrogate level of the control variate")
     ap.add_argument("--fp64-propagation", action="store_true",
@@ -441,7 +445,7 @@ def main():
                xcv["energy_shift_mha"], xcv["energy_shift_sem_mha"], xcv["err_per_sqrt_s_ratio"],
                xcv["live_frac"]))
         best_name = "mixed_exchange_cv"
-        if not a.no_mlp:
+        if not a.no_mlp and (a.mlp or a.workload != "c4"):
             mlp = run_path(ham, spec, T + a.warmup + a.steps, T + a.warmup, energy_mode=AFQ_ENERGY_SURROGATE,
                            prec_exchange=AFQ_PREC_FP64, surrogate_kind=AFQ_SURR_MLP, surrogate_train_steps=T,
                            surrogate_hidden=256, surrogate_epochs=a.epochs, subset_inv=a.subset_inv,
