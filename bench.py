@@ -100,6 +100,21 @@ def run_path(ham, spec, total_steps, warmup, **kw):
 HBM_PEAK_GBS = 8000.0
 
 
+def fused_prop(spec):
+    """Whether the split propagation runs as the fused k_prop_split (N <= 384)."""
+    return -(-spec["norb"] // 16) <= 8 * 3
+
+
+def fused_prop_shape(spec):
+    """(row tiles per wave, column tiles per block) of k_prop_split, as the library picks them."""
+    ncol = spec["nalpha"] + (0 if spec["walker_rhf"] else spec["nbeta"])
+    rt = -(-spec["norb"] // 16)
+    mt = -(-rt // 8)
+    ctot = -(-ncol // 8)
+    nblk = -(-ctot // 4)
+    return mt, -(-ctot // nblk)
+
+
 def stage_rooflines(spec, path, n_exact):
     """Roofline of each stage of one step (per launch for the kernels that launch
     several times).  Algorithmic FLOPs and HBM bytes per launch give two lower
@@ -165,23 +180,33 @@ def stage_rooflines(spec, path, n_exact):
         add("vhs", "k_vhs (FP64 MFMA)", W * 4.0 * T * G, 8.0 * G * T + W * 16.0 * (G + N * N), 1, "fp64",
             symbol="k_vhs")
     else:
-        # Horner steps on split operands: V (hi, lo: 8 Nq N bytes), u in, t0 and u out
-        # (4 planes of bf16: 8 ncol Nq bytes each); the last step adds the FP64 t0 and
-        # writes FP64
         Nq = (N + 15) // 16 * 16
         vb, ub = 8.0 * Nq * N, 8.0 * ncol * Nq
-        tay_split = W * (4 * (vb + 3 * ub) + (vb + ub + 32.0 * ms))
-        add("taylor", "k_taylor_h (5 Horner steps u = t0 + V u / p, 3-pass split BF16 MFMA)",
-            W * 40.0 * N * N * ncol, tay_split, 5, "bf16", symbol="void k_taylor_h<%d, %d>" % (
-                5 if -(-N // 160) * 160 < -(-N // 128) * 128 else 4, 1 if ncol <= 32 else 2 if ncol <= 64 else 4),
-            passes=3)
+        if fused_prop(spec):
+            # k_prop_split: both one-body factors and the 5 Horner steps in one kernel; the
+            # walker's orbitals are read and written once (FP64), V (hi, lo: 8 Nq N bytes)
+            # once per Horner step, D is shared by all walkers
+            mt, ct = fused_prop_shape(spec)
+            add("taylor", "k_prop_split (E (sum_k V^k/k!) E psi, fused, 3-pass split BF16 MFMA)",
+                W * (40.0 + 8.0) * N * N * ncol, W * (5 * vb + 32.0 * ms), 1, "bf16",
+                symbol="void k_prop_split<%d, %d>" % (mt, ct), passes=3)
+        else:
+            # Horner steps on split operands: V, u in, t0 and u out (4 planes of bf16:
+            # 8 ncol Nq bytes each); the last step adds the FP64 t0 and writes FP64
+            tay_split = W * (4 * (vb + 3 * ub) + (vb + ub + 32.0 * ms))
+            add("taylor", "k_taylor_h (5 Horner steps u = t0 + V u / p, 3-pass split BF16 MFMA)",
+                W * 40.0 * N * N * ncol, tay_split, 5, "bf16", symbol="void k_taylor_h<%d, %d>" % (
+                    5 if -(-N // 160) * 160 < -(-N // 128) * 128 else 4, 1 if ncol <= 32 else 2 if ncol <= 64 else 4),
+                passes=3)
         add("force_bias", "k_theta_split + k_force_bias_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * ncol * N * G,
             4.0 * G * ncol * Np + W * (16.0 * ms + 8.0 * ncol * Np + 16.0 * G), 1, "bf16",
             symbol="void k_force_bias_bf16<2>", passes=3)
         add("vhs", "k_fields_split + k_vhs_bf16<2> (3-pass split BF16 MFMA)", W * 4.0 * T * G,
             4.0 * G * T + W * (24.0 * G + 8.0 * ((N + 15) // 16 * 16) * N), 1, "bf16",
             symbol="void k_vhs_bf16<2>", passes=3)
-    if path.get("fb_vhs_prec", "fp64") == "fp64":
+    if path.get("fb_vhs_prec", "fp64") != "fp64" and fused_prop(spec):
+        pass   # the one-body factors run inside k_prop_split (the taylor stage)
+    elif path.get("fb_vhs_prec", "fp64") == "fp64":
         add("onebody", "k_expmul (E psi, FP64 MFMA)", W * 8.0 * N * N * ncol, W * 3 * 16.0 * ms, 2 * len(nsp),
             "fp64", symbol="k_expmul")
     else:
@@ -260,8 +285,10 @@ def micro(kind, workload, nwalk=1024, calls=8):
     from vmc_amd.afqmc import Context
     from vmc_amd.synthetic import workload as get_workload
     prec = {"exchange-bf16": AFQ_PREC_BF16, "exchange-bf16x3": AFQ_PREC_BF16X3, "exchange-fp64": AFQ_PREC_FP64,
-            "exchange-fp8": AFQ_PREC_FP8}[kind]
+            "exchange-fp8": AFQ_PREC_FP8, "prop-split": AFQ_PREC_BF16X3}[kind]
     h, spec = get_workload(workload)
+    if kind == "prop-split":
+        nwalk = spec["nwalk"]
     ctx = Context(h, nwalk, spec["walker_rhf"])
     ctx.prep_prop(h.phi_up, h.phi_dn, spec["dt"])
     rng = np.random.default_rng(0)
@@ -271,6 +298,23 @@ def micro(kind, workload, nwalk=1024, calls=8):
         dn = h.phi_dn + 0.02 * (rng.standard_normal(h.phi_dn.shape) + 1j * rng.standard_normal(h.phi_dn.shape))
         dets.append(h.serial_det(up, dn))
     ctx.set_walkers(np.array(dets))
+    if kind == "prop-split":
+        # the split propagation variants, timed by the autotuner on these walkers (it
+        # writes to scratch, so every call sees the same walkers and V)
+        from vmc_amd.afqmc import lib
+        import ctypes as C
+        ctx.set_precision(prec_fb=prec, prec_vhs=prec)
+        ctx.set_prop_precision(prec)
+        ctx.propagate(rng.standard_normal((nwalk, h.nchol)), 0.0)
+        ctx.set_walkers(np.array(dets))
+        out = []
+        for _ in range(calls):
+            ch, ms3 = C.c_int32(0), (C.c_double * 3)()
+            assert lib().afq_autotune(ctx.ptr, C.byref(ch), ms3) == 0
+            out.append(list(ms3))
+        print(json.dumps({"micro": kind, "workload": workload, "nwalk": nwalk,
+                          "ms_per_propagation": np.mean(out, axis=0).tolist()}))
+        return
     ctx.set_precision(prec_exchange=prec)
     ctx.measure()
     ctx.sync()
@@ -386,8 +430,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the 1 GiB plain-copy kernel (HBM counter calibration)")
-    ap.add_argument("--micro", choices=["exchange-bf16", "exchange-bf16x3", "exchange-fp64", "exchange-fp8"],
-                    help="kernel microbenchmark for profiling (1024 walkers, 8 measurements) instead of the bench")
+    ap.add_argument("--micro", choices=["exchange-bf16", "exchange-bf16x3", "exchange-fp64", "exchange-fp8",
+                                        "prop-split"],
+                    help="kernel microbenchmark for profiling instead of the bench: an exchange precision "
+                         "(1024 walkers, 8 measurements), or prop-split (the split propagation variants "
+                         "on the workload's walkers, 8 autotuner rounds)")
     a = ap.parse_args()
     if a.micro:
         micro(a.micro, a.workload)

@@ -259,13 +259,17 @@ int afq_set_energy_clip(afq_ctx* ctx, const double* center, double halfwidth);
 int afq_cv_stats(afq_ctx* ctx, double* out6);
 
 /* ---------------------------------------------------------------- tuning
- * Tile-shape autotuner (no counterpart in the reference): times the two Taylor-product
- * tile shapes of the context's propagation precision on its current walkers and keeps
- * the faster.  FP64: 1 = 64 x 128 real tiles, 2 = 128 x 128 complex tiles; split BF16:
- * 1 = 128-orbital row tiles, 2 = 160-orbital row tiles.  choice (nullable) receives 1
- * or 2; ms2 (nullable) the two times (ms per Taylor series). */
-int afq_autotune(afq_ctx* ctx, int32_t* choice, double* ms2);
-/* 0: by shape (default), 1 or 2: the tile shapes named at afq_autotune */
+ * Propagation-kernel autotuner (no counterpart in the reference): times the kernel
+ * variants of the context's propagation precision on its current walkers (without
+ * changing them) and keeps the fastest.  FP64, the five Taylor products: 1 = 64 x 128
+ * real tiles, 2 = 128 x 128 complex tiles.  Split BF16, the whole propagation
+ * E (sum V^k/k!) E: 1 = 128-orbital and 2 = 160-orbital row tiles with the one-body
+ * factors and each Taylor step as separate kernels, 3 = one fused kernel per block of
+ * columns (N <= 384 only).  choice (nullable) receives the variant; ms3 (nullable, 3
+ * entries) the times in ms per propagation (0 for a variant not tried). */
+int afq_autotune(afq_ctx* ctx, int32_t* choice, double* ms3);
+/* 0: by shape (default: the fused kernel where it applies), 1, 2 or 3: the variants
+ * named at afq_autotune (3 falls back to the shape rule where N > 384 or in FP64) */
 int afq_set_taylor_variant(afq_ctx* ctx, int32_t variant);
 
 /* ---------------------------------------------------------------- batched systems

@@ -370,7 +370,7 @@ class Batch:
         out = []
         for c in self.ctx:
             ch = C.c_int32(0)
-            ms = np.zeros(2)
+            ms = np.zeros(3)
             c._chk(lib().afq_autotune(c.ptr, C.byref(ch), dptr(ms)), "afq_autotune")
             out.append((ch.value, ms))
         return out

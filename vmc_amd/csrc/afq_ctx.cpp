@@ -1532,12 +1532,21 @@ int afq_autotune(afq_ctx* c, int32_t* choice, double* ms_out) {
   double best = 1e300;
   int pick = 1;
   const int keep = s.taylor_variant;
+  // split BF16: the whole propagation (both one-body factors and the series) into the
+  // scratch t1, so the walkers stay as they are; FP64: the five Taylor products
+  const bool split = s.prec_prop != AFQ_PREC_FP64;
+  const int nv = split && afq::prop_fused_fits(s) ? 3 : 2;
+  if (ms_out) ms_out[2] = 0.0;
+  auto run = [&]() {
+    if (split) afq::launch_prop_split_to(s, s.t1, c->stream);
+    else afq::launch_taylor(s, c->stream);
+  };
   try {
-    for (int v = 1; v <= 2; ++v) {
+    for (int v = 1; v <= nv; ++v) {
       s.taylor_variant = v;
-      afq::launch_taylor(s, c->stream);  // warm
+      run();  // warm
       hip_check(hipEventRecord(a, c->stream), "record");
-      for (int r = 0; r < 3; ++r) afq::launch_taylor(s, c->stream);
+      for (int r = 0; r < 3; ++r) run();
       hip_check(hipEventRecord(b, c->stream), "record");
       hip_check(hipEventSynchronize(b), "sync");
       float ms = 0.f;
@@ -1560,7 +1569,7 @@ int afq_autotune(afq_ctx* c, int32_t* choice, double* ms_out) {
 }
 
 int afq_set_taylor_variant(afq_ctx* c, int32_t v) {
-  if (!c || v < 0 || v > 2) return c ? fail(c, AFQ_ERR_ARG, "taylor variant must be 0, 1 or 2") : AFQ_ERR_ARG;
+  if (!c || v < 0 || v > 3) return c ? fail(c, AFQ_ERR_ARG, "taylor variant must be 0, 1, 2 or 3") : AFQ_ERR_ARG;
   c->s.taylor_variant = v;
   return AFQ_OK;
 }

@@ -107,8 +107,9 @@ struct DevState {
   const double* d_eshift = nullptr;
   const double* d_zeta = nullptr;
   bool graph_mode = false;
-  // Taylor tile shape: 0 by shape; FP64: 1 k_vmul (64 x 128 tiles), 2 k_taylor (128 x 128
-  // complex); split BF16: 1 128-row tiles, 2 160-row tiles
+  // Taylor kernels: 0 by shape; FP64: 1 k_vmul (64 x 128 tiles), 2 k_taylor (128 x 128
+  // complex); split BF16: 1 128-row tiles, 2 160-row tiles (one-body factors and Taylor
+  // steps as separate kernels), 3 the fused k_prop_split (the default where N <= 384)
   int taylor_variant = 0;
   int Gp = 0;               // G padded to a multiple of 8
   unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
@@ -163,6 +164,8 @@ void launch_propagate(const DevState& s, hipStream_t st);  // E psi, Taylor term
 void launch_expmul_pre(const DevState& s, hipStream_t st);
 void launch_taylor(const DevState& s, hipStream_t st);
 void launch_expmul_post(const DevState& s, hipStream_t st);
+bool prop_fused_fits(const DevState& s);                              // k_prop_split handles N
+void launch_prop_split_to(const DevState& s, double* dst, hipStream_t st);  // split path, result in dst
 void launch_green(const DevState& s, bool all_walkers, hipStream_t st);
 void launch_weight(const DevState& s, double eshift, bool update_weights, hipStream_t st);
 void launch_logO_from_logdet(const DevState& s, hipStream_t st);

@@ -598,6 +598,285 @@ __global__ __launch_bounds__(512) void k_taylor_h(const u16* Vh, const u16* Vl, 
     }
 }
 
+// The whole split-BF16 propagation of one walker's block of columns in one workgroup:
+//   x <- E (sum_{k<=5} V^k / k!) E x,   E = I + D,
+// (DQMC/DQMCWalker.cpp:424-431,482-491) with the Taylor sum by Horner's rule.  The block
+// (QB = 8*CT complex columns of the na + nb propagated ones, all N orbitals) stays in LDS
+// for the whole propagation, split (hi, lo) in the compact layout
+//   u[hl][re/im][chunk][q][16 orbitals]
+// and is the B operand of every product; the A operands (D = E - I, and V = the split
+// [-P_im | P_re] chunks of the VHS) are read straight from global memory into MFMA
+// fragments, since each wave owns its own rows (wave v: row tiles v, v+8, .., all CT
+// column tiles) and no other wave of the workgroup reads them.  So a product needs no
+// barrier; one step of the series costs two.  Eight waves, two per SIMD: the A fragments
+// run one or two K stages ahead in two or three register sets, and a product's first
+// stages are requested before the barrier that precedes it.  The first
+// factor t0 = E x is kept in FP32 registers at each lane's output positions; Horner's
+// steps u <- t0 + (V u) sqrt(dt)/p go back to LDS split, and the result leaves as FP64
+// after the last E.  The blocks of one walker run back to back on one XCD, so V and D
+// stream from HBM about once per product for all of them.  Walkers at or below the
+// propagation threshold are skipped.
+constexpr int PS_WAVES = 8;
+constexpr int PS_MT_MAX = 3;   // row tiles per wave: N <= 16 * PS_WAVES * PS_MT_MAX = 384
+template <int MT, int CT>
+struct PropShape {
+  static constexpr int QB = 8 * CT;                     // complex columns per block
+  // orbital chunks of 16 held in LDS: the N orbitals (at most PS_WAVES MT chunks) and
+  // zero chunks for the stages a product's loop adds past its end
+  static constexpr int NCH = PS_WAVES * MT + 4;
+  static constexpr int PLANE = NCH * QB * 16 + 8;       // u16 per plane (+ 16 B against bank aliasing)
+  static constexpr int LDS_U16 = 4 * PLANE;
+};
+template <int MT, int CT>
+__global__ __launch_bounds__(64 * PS_WAVES, 1) void k_prop_split(const u16* __restrict__ Vh, const u16* __restrict__ Vl,
+                                                                 const u16* __restrict__ Dh, const u16* __restrict__ Dl,
+                                                                 const double* src, double* dst, const double* weights,
+                                                                 int N, int Nd, int na, int nb, int64_t offb,
+                                                                 int64_t wstride, float sq, int nblk, int W) {
+  using S = PropShape<MT, CT>;
+  constexpr int QB = S::QB, PLANE = S::PLANE, NT = 64 * PS_WAVES;
+  __shared__ __attribute__((aligned(16))) u16 u[S::LDS_U16];   // [hi re, hi im, lo re, lo im]
+  const TileId tl = tile_id(nblk, 1, W, 1);
+  const int w = tl.z, blk = tl.x;
+  if (weights[w] <= 1e-8) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int RT = (N + 15) >> 4;            // row tiles of 16 orbitals
+  const int nch = RT;                      // V chunks (16 orbitals each)
+  const int nks = (N + 31) >> 5;           // D steps (32 orbitals each)
+  const int ncolc = na + nb, q0 = blk * QB;
+  const int64_t vper = (int64_t)nch * N * 32;
+  const u16* vh = Vh + (int64_t)w * vper;
+  const u16* vl = Vl + (int64_t)w * vper;
+  const double* xs = src + (int64_t)w * wstride;
+  double* xd = dst + (int64_t)w * wstride;
+
+  for (int e = tid; e < S::LDS_U16 / 8; e += NT) reinterpret_cast<uint4*>(u)[e] = uint4{0u, 0u, 0u, 0u};
+
+  // this lane's output positions: rows 16 rt + 4 (lane >> 4) + r of column 16 j + (lane & 15)
+  const int cl = lane & 15, kh = lane >> 4;
+  int rt[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) rt[m] = wave + PS_WAVES * m;
+  // FP64 offset of (row, this lane's column of column tile j) in the walker: cb[j] + row rs[j]
+  int cb[CT], rs[CT];
+  auto col_setup = [&](int qb) {
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int q = qb + 8 * j + (cl >> 1), comp = cl & 1;
+      cb[j] = q < na ? 2 * q + comp : (int)(2 * (offb + q - na)) + comp;
+      rs[j] = q < ncolc ? (q < na ? 2 * na : 2 * nb) : -1;   // -1: padding column
+    }
+  };
+  col_setup(q0);
+  auto valid = [&](int row, int j) { return row < N && rs[j] >= 0; };
+  // LDS offset (u16) of 4 consecutive orbitals row.. of column q in plane (hl, comp)
+  auto uoff = [&](int hl, int comp, int row, int q) { return (2 * hl + comp) * PLANE + ((row >> 4) * QB + q) * 16 + (row & 15); };
+
+  float T[MT][CT][4];
+  f32x4 acc[MT][CT];
+  __syncthreads();
+  // x: FP32 copy at the output positions, split into LDS
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int rb = 16 * rt[m] + 4 * kh;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        T[m][j][r] = (rt[m] < RT && valid(rb + r, j)) ? (float)xs[cb[j] + (int64_t)(rb + r) * rs[j]] : 0.f;
+    }
+  auto put = [&]() {   // T (or its Horner update in T) -> u, split
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      if (rt[m] >= RT) continue;   // wave-uniform
+#pragma unroll
+      for (int j = 0; j < CT; ++j) {
+        const int c = 16 * j + cl, rb = 16 * rt[m] + 4 * kh;
+        unsigned oh[4], ol[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float t = rb + r < N ? T[m][j][r] : 0.f;   // orbital pads of u stay zero
+          const u16 h = f2bf_bits(t);
+          oh[r] = h;
+          ol[r] = f2bf_bits(t - bf_bits2f(h));
+        }
+        *reinterpret_cast<uint2*>(u + uoff(0, c & 1, rb, c >> 1)) = uint2{oh[0] | (oh[1] << 16), oh[2] | (oh[3] << 16)};
+        *reinterpret_cast<uint2*>(u + uoff(1, c & 1, rb, c >> 1)) = uint2{ol[0] | (ol[1] << 16), ol[2] | (ol[3] << 16)};
+      }
+    }
+  };
+  auto zero_acc = [&]() {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int j = 0; j < CT; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  };
+  auto mma3 = [&](const bf16x8* ah, const bf16x8* al, const bf16x8* bh, const bf16x8* bl) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      if (rt[m] >= RT) continue;   // wave-uniform
+#pragma unroll
+      for (int j = 0; j < CT; ++j) {
+        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bh[j], acc[m][j], 0, 0, 0);
+        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[m], bl[j], acc[m][j], 0, 0, 0);
+        acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[m], bh[j], acc[m][j], 0, 0, 0);
+      }
+    }
+  };
+  // A fragments straight from global memory into one of
+  // NS register sets, stage k in set k % NS: three sets (loads two stages ahead) where
+  // the registers allow, else two
+  // (a wave-uniform row-tile base plus one 32-bit per-lane offset per operand, so that the
+  // loads take a scalar base and the offsets cost one VGPR each)
+  constexpr int NS = MT * CT >= 9 ? 2 : 3;
+  bf16x8 Ah[NS][MT], Al[NS][MT];
+  // The loads are unconditional (no branch, so the compiler can count them when it waits
+  // for a stage): rows past N, and the row tiles a wave does not have, read row N - 1
+  // instead, and the MFMA rows they feed never reach LDS or the output.
+  unsigned doff[MT], voff[MT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    const int row = min(16 * rt[m] + cl, N - 1);
+    doff[m] = (unsigned)(row * Nd + 8 * kh);
+    voff[m] = (unsigned)(row * 32 + 8 * kh);
+  }
+  auto loadA = [&](const u16* hp, const u16* lp, const unsigned* off, int64_t koffs, bf16x8* h, bf16x8* l) {
+    const u16* hb = hp + koffs;   // wave-uniform
+    const u16* lb = lp + koffs;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      h[m] = *reinterpret_cast<const bf16x8*>(hb + off[m]);
+      l[m] = *reinterpret_cast<const bf16x8*>(lb + off[m]);
+    }
+  };
+  // D product: 32-orbital stages, A rows from Dh/Dl (pitch Nd), B = u without sign flips
+  auto loadD = [&](int ks, int set) { loadA(Dh, Dl, doff, 32 * (int64_t)ks, Ah[set], Al[set]); };
+  auto dstep = [&](int ks, int set) {
+    const int k = 32 * ks + 8 * kh;
+    bf16x8 bh[CT], bl[CT];
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int c = 16 * j + cl;
+      bh[j] = *reinterpret_cast<const bf16x8*>(u + uoff(0, c & 1, k, c >> 1));
+      bl[j] = *reinterpret_cast<const bf16x8*>(u + uoff(1, c & 1, k, c >> 1));
+    }
+    mma3(Ah[set], Al[set], bh, bl);
+  };
+  // V product: a chunk's 32 K are [16 orbitals of -P_im | 16 of P_re], against B column
+  // (q, re) = [u_re | -u_im] and (q, im) = [u_im | u_re]
+  const int sgn = kh >> 1, koff = 8 * (kh & 1);
+  const unsigned nmask = ((cl & 1) == 0 && sgn) ? 0x80008000u : 0u;   // lanes holding -u_im
+  auto loadV = [&](int ch, int set) { loadA(vh, vl, voff, (int64_t)ch * N * 32, Ah[set], Al[set]); };
+  auto vstep = [&](int ch, int set) {
+    bf16x8 bh[CT], bl[CT];
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int c = 16 * j + cl, comp = c & 1;
+      const int o = uoff(0, comp ^ sgn, 16 * ch + koff, c >> 1);
+      uint4 xh = *reinterpret_cast<const uint4*>(u + o);
+      uint4 xl = *reinterpret_cast<const uint4*>(u + o + 2 * PLANE);
+      xh = uint4{xh.x ^ nmask, xh.y ^ nmask, xh.z ^ nmask, xh.w ^ nmask};
+      xl = uint4{xl.x ^ nmask, xl.y ^ nmask, xl.z ^ nmask, xl.w ^ nmask};
+      bh[j] = *reinterpret_cast<const bf16x8*>(&xh);
+      bl[j] = *reinterpret_cast<const bf16x8*>(&xl);
+    }
+    mma3(Ah[set], Al[set], bh, bl);
+  };
+  // one product over n stages whose stages 0 .. NS-2 are already requested (sets 0 ..
+  // NS-2).  The loop is unrolled by NS (every set index a constant) and has no branch at
+  // all, so the compiler can count the loads in flight when it waits for a stage: n is
+  // rounded up to a multiple of NS, and the stages past n re-read the last A stage
+  // against the zero orbital pads of u (NCH leaves room for them), adding nothing.
+  auto product = [&](int n, auto load, auto step) {
+    zero_acc();
+    const int np = (n + NS - 1) / NS * NS;
+    for (int k = 0; k < np; k += NS) {
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        load(min(k + i + NS - 1, n - 1), (i + NS - 1) % NS);
+        step(k + i, i);
+      }
+    }
+  };
+  auto preD = [&]() {
+#pragma unroll
+    for (int i = 0; i + 1 < NS; ++i) loadD(min(i, nks - 1), i);
+  };
+  auto preV = [&]() {
+#pragma unroll
+    for (int i = 0; i + 1 < NS; ++i) loadV(min(i, nch - 1), i);
+  };
+
+  put();
+  preD();
+  __syncthreads();
+  product(nks, loadD, dstep);    // t0 = x + D x
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int j = 0; j < CT; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) T[m][j][r] += acc[m][j][r];
+  preV();
+  __syncthreads();
+  put();
+  __syncthreads();
+  for (int p = 5; p >= 1; --p) {
+    product(nch, loadV, vstep);
+    if (p > 1) preV();
+    else preD();
+    __syncthreads();
+    const float f = sq / (float)p;
+    if (p > 1) {
+      // u <- t0 + (V u) f: T keeps t0, the update goes to LDS through T's slot
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int j = 0; j < CT; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float t0 = T[m][j][r];
+            T[m][j][r] = t0 + acc[m][j][r] * f;
+            acc[m][j][r] = t0;
+          }
+      put();
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int j = 0; j < CT; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) T[m][j][r] = acc[m][j][r];
+    } else {
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int j = 0; j < CT; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) T[m][j][r] += acc[m][j][r] * f;
+      put();
+    }
+    __syncthreads();
+  }
+  product(nks, loadD, dstep);    // E y = y + D y
+  // the column offsets again from scratch (opaque to the compiler), so that the
+  // prologue's addresses are not kept live through the whole kernel
+  int q0o = q0;
+  asm volatile("" : "+v"(q0o));
+  col_setup(q0o);
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+    if (rt[m] >= RT) continue;
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int rb = 16 * rt[m] + 4 * kh;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (valid(rb + r, j)) xd[cb[j] + (int64_t)(rb + r) * rs[j]] = (double)T[m][j][r] + (double)acc[m][j][r];
+    }
+  }
+}
+
 // V P -> the split [-P_im | P_re] chunks of k_taylor_h (used when the VHS ran in FP64)
 __global__ void k_v_split(const double* P, u16* Vh, u16* Vl, const double* weights, int N, int Nq, int W) {
   const int nch = Nq >> 4;
@@ -805,10 +1084,57 @@ static void launch_onebody_split(const DevState& s, const double* src, double* d
   else onebody_split<4>(s, src, dst, mode, st);
 }
 
+// ---- the fused split propagation (k_prop_split)
+bool prop_fused_fits(const DevState& s) { return (s.N + 15) / 16 <= PS_WAVES * PS_MT_MAX; }
+static bool prop_fused(const DevState& s) {
+  if (s.prec_prop == AFQ_PREC_FP64 || !prop_fused_fits(s)) return false;
+  return s.taylor_variant == 0 || s.taylor_variant == 3;
+}
+template <int MT, int CT>
+static void prop_split_mc(const DevState& s, const double* src, double* dst, hipStream_t st) {
+  const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
+  const int nblk = (int)cdivu(na + nb, 8 * CT);
+  hipLaunchKernelGGL((k_prop_split<MT, CT>), dim3((unsigned)(nblk * s.W)), dim3(64 * PS_WAVES), 0, st, s.Vh, s.Vl,
+                     s.Dh, s.Dl, src, dst, s.weights, s.N, onebody_pitch(s.N), na, nb, s.det_off[1], 2 * s.ms,
+                     (float)sqrt(s.dt), nblk, s.W);
+}
+template <int MT>
+static void prop_split_m(const DevState& s, int ct, const double* src, double* dst, hipStream_t st) {
+  switch (ct) {
+    case 1: prop_split_mc<MT, 1>(s, src, dst, st); break;
+    case 2: prop_split_mc<MT, 2>(s, src, dst, st); break;
+    case 3: prop_split_mc<MT, 3>(s, src, dst, st); break;
+    default: prop_split_mc<MT, 4>(s, src, dst, st); break;
+  }
+}
+// dst <- E (sum_{k<=5} V^k / k!) E src over the propagated columns (dst may be src)
+static void launch_prop_fused(const DevState& s, const double* src, double* dst, hipStream_t st) {
+  if (s.prec_vhs == AFQ_PREC_FP64) launch_v_split(s, st);
+  // column tiles of 8 complex columns, at most 4 per block, spread evenly over the blocks
+  const int mt = (int)cdivu(cdivu(s.N, 16), PS_WAVES);
+  const int ctot = (int)cdivu(s.det_n[0] + (s.ndets_prop > 1 ? s.det_n[1] : 0), 8);
+  const int nblk = (int)cdivu(ctot, 4), ct = (int)cdivu(ctot, nblk);
+  if (mt <= 1) prop_split_m<1>(s, ct, src, dst, st);
+  else if (mt == 2) prop_split_m<2>(s, ct, src, dst, st);
+  else prop_split_m<3>(s, ct, src, dst, st);
+}
+static void launch_taylor_split(const DevState& s, hipStream_t st);
+// The split path's whole propagation with the result in dst (psi itself in a step;
+// scratch when the autotuner times it)
+void launch_prop_split_to(const DevState& s, double* dst, hipStream_t st) {
+  if (prop_fused(s)) {
+    launch_prop_fused(s, s.psi, dst, st);
+    return;
+  }
+  launch_onebody_split(s, s.psi, s.t0, 0, st);
+  launch_taylor_split(s, st);
+  launch_onebody_split(s, s.accb, dst, 1, st);
+}
+
 // psi <- E (sum_{k<=5} V^k / k!) E psi  for every live walker
 void launch_expmul_pre(const DevState& s, hipStream_t st) {
   if (s.prec_prop != AFQ_PREC_FP64) {
-    launch_onebody_split(s, s.psi, s.t0, 0, st);
+    if (!prop_fused(s)) launch_onebody_split(s, s.psi, s.t0, 0, st);
     return;
   }
   const int64_t ws = 2 * s.ms;
@@ -844,7 +1170,7 @@ static void taylor_split_cols(const DevState& s, hipStream_t st) {
 static void launch_taylor_split(const DevState& s, hipStream_t st) {
   if (s.prec_vhs == AFQ_PREC_FP64) launch_v_split(s, st);
   int v = s.taylor_variant;
-  if (v == 0) {
+  if (v != 1 && v != 2) {
     const int64_t p4 = (int64_t)cdivu(s.N, 128) * 128, p5 = (int64_t)cdivu(s.N, 160) * 160;
     v = p5 < p4 ? 2 : 1;
   }
@@ -859,10 +1185,11 @@ void launch_taylor(const DevState& s, hipStream_t st) {
   const double sq = sqrt(s.dt);
   double* src = s.t0;
   double* dst = s.t1;
-  const bool wide = s.taylor_variant == 0 ? n2a + n2b > 128 : s.taylor_variant == 2;
+  const bool wide = (s.taylor_variant == 1 || s.taylor_variant == 2) ? s.taylor_variant == 2 : n2a + n2b > 128;
   const int na = s.det_n[0], nb = s.ndets_prop > 1 ? s.det_n[1] : 0;
   if (s.prec_prop != AFQ_PREC_FP64) {
-    launch_taylor_split(s, st);
+    if (prop_fused(s)) launch_prop_fused(s, s.psi, s.psi, st);
+    else launch_taylor_split(s, st);
     return;
   }
   for (int p = 1; p <= 5; ++p) {
@@ -884,7 +1211,7 @@ void launch_taylor(const DevState& s, hipStream_t st) {
 }
 void launch_expmul_post(const DevState& s, hipStream_t st) {
   if (s.prec_prop != AFQ_PREC_FP64) {
-    launch_onebody_split(s, s.accb, s.psi, 1, st);
+    if (!prop_fused(s)) launch_onebody_split(s, s.accb, s.psi, 1, st);
     return;
   }
   const int64_t ws = 2 * s.ms;
