@@ -1237,7 +1237,7 @@ constexpr int GN_MAX = 64;
 //  2. k_invert: S^-1 and log det S per (walker, slot) by Gauss-Jordan elimination with
 //     partial pivoting, the matrix held in registers (lane = row, wave q = complex
 //     columns CPW q .. CPW q + CPW - 1), so a workgroup needs only 3 KB of LDS;
-//  3. k_theta: theta = psi S^-1 (FP64 MFMA, S^-1 read from Sg) and the one-body
+//  3. k_theta_r: theta = psi S^-1 (FP64 MFMA, S^-1 held in registers) and the one-body
 //     partial e1 = sum theta o (h1 Phi).
 struct GreenArgs {
   const double* psi; double* theta; double* logdet; double* e1; const double* weights; double* Sg;
@@ -1365,47 +1365,101 @@ __global__ __launch_bounds__(64 * (64 / CPW), CPW == 16 ? 3 : 2) void k_invert(G
   }
 }
 
-__global__ __launch_bounds__(256, 2) void k_theta(GreenArgs a) {
-  using ET = MmaF64<64, 128, 2, 2, false, true>;
-  __shared__ double stage[ET::LDS_DOUBLES];
+// theta = psi S^-1 with S^-1 held in registers: the real embedding B = [[S_re, S_im],
+// [-S_im, S_re]] of S^-1 (2n x 2n, zero-padded to 128 x 128) is split by columns over
+// eight waves (wave v: columns 16v .. 16v+15, as MFMA B fragments, 32 doubles a lane),
+// and the rows of psi stream through LDS 16 at a time (double-buffered, fetched into
+// registers one tile ahead, one barrier per row tile), so each row tile is 32 FP64 MFMA
+// K steps per wave with no other traffic.  Also the one-body partial
+// e1 = sum theta o (h1 Phi).
+constexpr int TR_KS = 2 * GN_MAX / 4;   // K steps of 4
+constexpr int TR_LDA = 20;              // LDS row pitch of a staged A tile [k][16 rows] (doubles)
+constexpr int TR_NT = 512;
+__global__ __launch_bounds__(TR_NT, 2) void k_theta_r(GreenArgs a) {
+  __shared__ double As[2][2 * GN_MAX * TR_LDA];
   __shared__ double red[16];
   const int w = blockIdx.x, si = blockIdx.y;
   if (!a.all && a.weights[w] <= 1e-8) return;
   const Slot sl = si == 0 ? a.s0 : a.s1;
   const int n = sl.n, N = a.N, n2 = 2 * n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double* psi = a.psi + (int64_t)w * 2 * a.ms + 2 * (sl.det == 0 ? a.det_off0 : a.det_off1);
   const double* hphi = sl.phi == 0 ? a.hphi0 : a.hphi1;
   const double* Si = a.Sg + ((int64_t)w * a.nslots + si) * SG_MAT;
-  double e1re = 0.0, e1im = 0.0;
   double* th = a.theta + (int64_t)w * 2 * a.Ktot + 2 * sl.koff;
-  for (int r0 = 0; r0 < N; r0 += 64) {
-    ET e;
-    e.run(
-        [&](int r, int k) {
-          const int rr = r0 + r;
-          if (rr >= N || k >= n2) return 0.0;
-          return k < n ? psi[(int64_t)rr * n2 + 2 * k] : psi[(int64_t)rr * n2 + 2 * (k - n) + 1];
-        },
-        [&](int k, int c) {
-          if (k >= n2 || c >= n2) return 0.0;
-          const int j = c >> 1, comp = c & 1;
-          if (k < n) return Si[k * SG_LD + 2 * j + comp];
-          const double* v = Si + (k - n) * SG_LD + 2 * j;
-          return comp ? v[0] : -v[1];
-        },
-        n2, stage);
-    e.store([&](int r, int c, double v) {
-      const int rr = r0 + r;
+  const int c = 16 * wave + (lane & 15);   // this lane's output column
+  // B fragments: lane holds B[k = 4 ks + (lane >> 4)][c]
+  double b[TR_KS];
+#pragma unroll
+  for (int ks = 0; ks < TR_KS; ++ks) {
+    const int k = 4 * ks + (lane >> 4);
+    double v = 0.0;
+    if (k < n2 && c < n2) {
+      const int j = c >> 1, comp = c & 1;
+      if (k < n) v = Si[k * SG_LD + 2 * j + comp];
+      else v = comp ? Si[(k - n) * SG_LD + 2 * j] : -Si[(k - n) * SG_LD + 2 * j + 1];
+    }
+    b[ks] = v;
+  }
+  // A tile: psi rows r0 .. r0+15 as [k][row], k < n the real and k >= n the imaginary
+  // parts; k >= 2n stays zero
+  const int ntile = (N + 15) >> 4;
+  for (int e = tid; e < 2 * 2 * GN_MAX * TR_LDA; e += TR_NT) (&As[0][0])[e] = 0.0;
+  __syncthreads();
+  // a tile's elements e = tid + 512 i (at most 16 x 128 = 4 a thread): fetched into
+  // registers one tile ahead, written to LDS after this tile's MFMAs; the element ->
+  // (global offset within the tile, LDS offset) map is the same for every tile
+  int goff[4], loff[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = tid + TR_NT * i, r = e / n2, q = e - r * n2;
+    const int k = (q & 1) ? n + (q >> 1) : (q >> 1);
+    goff[i] = e < 16 * n2 ? e : -1;   // = r n2 + q
+    loff[i] = k * TR_LDA + r;
+  }
+  double ra[4];
+  auto fetch = [&](int r0) {
+    const int lim = (N - r0) * n2;   // elements of the tile's rows below N
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      ra[i] = (goff[i] >= 0 && goff[i] < lim) ? psi[(int64_t)r0 * n2 + goff[i]] : 0.0;
+  };
+  auto put = [&](double* dstA) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (goff[i] >= 0) dstA[loff[i]] = ra[i];
+  };
+  double e1re = 0.0, e1im = 0.0;
+  fetch(0);
+  put(As[0]);
+  __syncthreads();
+  for (int rt = 0; rt < ntile; ++rt) {
+    const double* A = As[rt & 1];
+    if (rt + 1 < ntile) fetch(16 * (rt + 1));
+    double hv[4];   // h1 Phi at this tile's output positions, for e1
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rr = 16 * rt + (lane >> 4) + 4 * r;
+      hv[r] = (rr < N && c < n2) ? hphi[(int64_t)rr * n + (c >> 1)] : 0.0;
+    }
+    f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < TR_KS; ++ks)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(4 * ks + (lane >> 4)) * TR_LDA + (lane & 15)], b[ks], acc, 0, 0, 0);
+    if (rt + 1 < ntile) put(As[(rt + 1) & 1]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rr = 16 * rt + (lane >> 4) + 4 * r;
       if (rr < N && c < n2) {
-        th[(int64_t)rr * n2 + c] = v;
-        const double h = hphi[(int64_t)rr * n + (c >> 1)];
-        if (c & 1) e1im += v * h; else e1re += v * h;
+        th[(int64_t)rr * n2 + c] = acc[r];
+        if (c & 1) e1im += acc[r] * hv[r]; else e1re += acc[r] * hv[r];
       }
-    });
+    }
+    __syncthreads();
   }
   e1re = block_sum(e1re, red);
   e1im = block_sum(e1im, red);
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     a.e1[2 * ((int64_t)w * a.nslots + si)] = e1re;
     a.e1[2 * ((int64_t)w * a.nslots + si) + 1] = e1im;
   }
@@ -1423,7 +1477,7 @@ void launch_green(const DevState& s, bool all, hipStream_t st) {
     hipLaunchKernelGGL(k_overlap, dim3(gx), dim3(256), 0, st, a, si, gx);
   }
   hipLaunchKernelGGL(k_invert<16>, dim3(s.W, s.nslots), dim3(256), 0, st, a);   // 4 waves: faster than 8 at n = 60
-  hipLaunchKernelGGL(k_theta, dim3(s.W, s.nslots), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_theta_r, dim3(s.W, s.nslots), dim3(TR_NT), 0, st, a);
 }
 
 // ------------------------------------------------------------------ weights
