@@ -271,6 +271,11 @@ int afq_autotune(afq_ctx* ctx, int32_t* choice, double* ms3);
 /* 0: by shape (default: the fused kernel where it applies), 1, 2 or 3: the variants
  * named at afq_autotune (3 falls back to the shape rule where N > 384 or in FP64) */
 int afq_set_taylor_variant(afq_ctx* ctx, int32_t variant);
+/* One-pass BF16 exchange kernel (no counterpart in the reference): 0 by shape (default:
+ * the pair-wise kernel, which keeps a walker's theta in LDS and forms f_ij f_ji in
+ * registers, wherever every slot has at most 64 electrons and N <= 512), 1 the tiled
+ * kernel (128 x 128 output tiles, pair sums through LDS), 2 the pair-wise kernel. */
+int afq_set_exchange_variant(afq_ctx* ctx, int32_t variant);
 
 /* ---------------------------------------------------------------- batched systems
  * Many independent populations of different sizes on one GPU (no counterpart in the

@@ -202,3 +202,26 @@ def test_fused_propagation(N, na, nb):
     r_sep64 = np.max(np.abs(dsep - d64)) / np.max(np.abs(d64))
     print(N, na, nb, "fused vs separate %.2e  fused vs fp64 %.2e  separate vs fp64 %.2e" % (r_sep, r_64, r_sep64))
     assert r_64 < 3e-6 and r_sep64 < 3e-6 and r_sep < 3e-6
+
+
+@pytest.mark.parametrize("name", ["uhf_uhf", "rhf_rhf", "c0_rhf", "uhf_33_31", "uhf_n60", "c1_small"])
+def test_exchange_pairwise_matches_tiled(name):
+    """The one-pass BF16 exchange as the pair-wise kernel (theta in LDS, f_ij f_ji in
+    registers) against the tiled kernel (pair sums through LDS): the same BF16 operands
+    and FP32 products, summed in another order, so the local energies agree to FP32
+    rounding (1e-6 of |E_L|), on every walker and on a live subset."""
+    W = 7
+    h, wrhf, ctx, orc = build(name, W)
+    dets = perturbed_walkers(h, W, wrhf, np.random.default_rng(21), 0.05)
+    ctx.set_walkers(dets)
+    ctx.set_precision(prec_exchange=AFQ_PREC_BF16)
+    out = {}
+    for v in (1, 2):
+        ctx.set_exchange_variant(v)
+        out[v], _ = ctx.measure()
+    ctx.set_exchange_variant(0)
+    e0, _ = ctx.measure()
+    d = np.max(np.abs(out[2] - out[1]) / np.abs(out[1]))
+    print(name, "pair-wise vs tiled %.2e" % d)
+    assert d < 1e-6
+    assert np.array_equal(e0, out[2])

@@ -162,6 +162,10 @@ class Context:
     def set_prop_precision(self, prec_prop=AFQ_PREC_FP64):
         self._chk(lib().afq_set_prop_precision(self.ptr, prec_prop), "afq_set_prop_precision")
 
+    def set_exchange_variant(self, variant):
+        """One-pass BF16 exchange kernel: 0 by shape, 1 tiled, 2 pair-wise."""
+        self._chk(lib().afq_set_exchange_variant(self.ptr, int(variant)), "afq_set_exchange_variant")
+
     def set_nchol_ene(self, n):
         self._chk(lib().afq_set_nchol_ene(self.ptr, int(n)), "afq_set_nchol_ene")
 

@@ -1568,6 +1568,12 @@ int afq_autotune(afq_ctx* c, int32_t* choice, double* ms_out) {
   AFQ_GUARD_END(c)
 }
 
+int afq_set_exchange_variant(afq_ctx* c, int32_t v) {
+  if (!c || v < 0 || v > 2) return c ? fail(c, AFQ_ERR_ARG, "exchange variant must be 0, 1 or 2") : AFQ_ERR_ARG;
+  c->s.xc_variant = v;
+  return AFQ_OK;
+}
+
 int afq_set_taylor_variant(afq_ctx* c, int32_t v) {
   if (!c || v < 0 || v > 3) return c ? fail(c, AFQ_ERR_ARG, "taylor variant must be 0, 1, 2 or 3") : AFQ_ERR_ARG;
   c->s.taylor_variant = v;

@@ -111,6 +111,9 @@ struct DevState {
   // complex); split BF16: 1 128-row tiles, 2 160-row tiles (one-body factors and Taylor
   // steps as separate kernels), 3 the fused k_prop_split (the default where N <= 384)
   int taylor_variant = 0;
+  // one-pass BF16 exchange: 0 by shape (the pair-wise k_exchange_pw where it applies),
+  // 1 the tiled k_exchange_bf16, 2 k_exchange_pw
+  int xc_variant = 0;
   int Gp = 0;               // G padded to a multiple of 8
   unsigned short* Lt_hi = nullptr; unsigned short* Lt_lo = nullptr;  // packed L transposed, [t][Gp]
   unsigned short* Fb_hi = nullptr; unsigned short* Fb_lo = nullptr;  // fields, [w][re/im][Gp]

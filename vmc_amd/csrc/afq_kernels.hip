@@ -2173,13 +2173,223 @@ __global__ __launch_bounds__(256, NS == 1 ? 4 : 2) void k_exchange_bf16(const u1
   }
 }
 
+// One-pass BF16 exchange with the pair sums sum_ij f_ij f_ji in registers.  A workgroup
+// takes one (live walker, slot) and a chunk of the Cholesky vectors: the walker's theta
+// for the slot (both components, rows zero-padded to 16 NB, K to a multiple of 32)
+// stays in LDS for the whole chunk, and each wave takes its own g's, reading R_g's rows
+// straight into MFMA fragments.  For every pair of 16-row blocks I <= J of
+// f_c = R_g theta_c^T (c = re, im) the wave forms X_c = f_c[I,J] (A = R_I,
+// B = theta_c,J) and Y_c = f_c[J,I]^T (A = theta_c,I, B = R_J), which share a register
+// layout, so the pair's sum_ab f[I,J]_ab f[J,I]_ba is an elementwise product of
+// registers (off-diagonal pairs count twice, and the diagonal blocks are formed twice:
+// 25% more MFMA at NB = 4) with no LDS round trip.  Padded rows of theta are zero, so
+// the clamped (finite) R rows and K columns past the data contribute nothing.  The
+// loads have no branches and run one K stage ahead.  Eight waves, two per
+// SIMD: the two waves of a pair take the same g's and split the block pairs, so each
+// holds half the accumulators.  The per-g-tile partials go to LDS and leave at the end
+// in the layout of k_exchange_bf16's.
+constexpr int PW_WAVES = 8, PW_STREAMS = 4, PW_KMAX = 512, PW_TMAX = 160;
+template <int NB>
+__global__ __launch_bounds__(64 * PW_WAVES, 1) void k_exchange_pw(const u16* __restrict__ Rh,
+                                                                 const u16* __restrict__ Th, const int* sel,
+                                                                 double* Xp, int Np, int GE, int nw, int W, int G,
+                                                                 Slot s0, Slot s1, int ctot, int ngt_cap, int gt,
+                                                                 int tpc, int nz, int nchunk) {
+  constexpr int ROWS = 16 * NB;
+  constexpr int NPAIR = NB * (NB + 1) / 2, NPH = (NPAIR + 1) / 2;   // block pairs, per wave of a pair
+  __shared__ __attribute__((aligned(16))) u16 Ts[2 * ROWS * (PW_KMAX + 8)];
+  __shared__ double part[2][PW_TMAX][2];
+  const int Kp = (Np + 31) & ~31, nks = Kp >> 5, LDK = Kp + 8;
+  const TileId tl = tile_id(nw * nz, 1, nchunk, 1);
+  const int zs = tl.x % nz, l = tl.x / nz, chunk = tl.z;
+  const Slot sl = zs == 0 ? s0 : s1;
+  const int n = sl.n;
+  const int coff = (zs == 0 && sl.koff == s0.koff) ? 0 : s0.n;
+  const int64_t roff = (zs == 0 && sl.koff == s0.koff) ? 0 : (int64_t)G * s0.n * Np;
+  const int w = sel ? sel[l] : l;
+  const int ngt = (GE + gt - 1) / gt, t0 = chunk * tpc, t1 = min(ngt, t0 + tpc);
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int q = wave & 1, u = wave >> 1;   // block-pair half, g stream
+  const int cl = lane & 15, kh = lane >> 4;
+  // theta of (w, slot) -> LDS [comp][row][LDK], zero outside the data
+  for (int e = tid; e < 2 * ROWS * (LDK / 8); e += 64 * PW_WAVES) {
+    const int k8 = e % (LDK / 8), rr = e / (LDK / 8), comp = rr / ROWS, j = rr - comp * ROWS;
+    uint4 v = uint4{0u, 0u, 0u, 0u};
+    if (j < n && 8 * k8 < Np)
+      v = *reinterpret_cast<const uint4*>(Th + (((int64_t)w * 2 + comp) * (ctot / 2) + coff + j) * Np + 8 * k8);
+    *reinterpret_cast<uint4*>(Ts + (comp * ROWS + j) * LDK + 8 * k8) = v;
+  }
+  for (int e = tid; e < 2 * PW_TMAX * 2; e += 64 * PW_WAVES) (&part[0][0][0])[e] = 0.0;
+  __syncthreads();
+  // this wave's g's: g-tiles t = t0 + u, t0 + u + PW_STREAMS, ..., gt g's each; both waves
+  // of a stream take the same g's and split the block pairs (p even / odd)
+  const int ntw = t1 - t0 > u ? (t1 - t0 - u + PW_STREAMS - 1) / PW_STREAMS : 0;
+  const int ng = ntw * gt;
+  auto g_of = [&](int gi) { return min((t0 + u + PW_STREAMS * (gi / gt)) * gt + gi % gt, GE - 1); };
+  const u16* Rs = Rh + roff;
+  int roffs[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) roffs[b] = min(16 * b + cl, n - 1) * Np;
+  // R and theta fragments run one stage ahead in two register sets
+  bf16x8 Ra[2][NB], Ta[2][2][NB];
+  auto loadR = [&](int g, int ks, int set) {
+    const u16* base = Rs + (int64_t)g * n * Np;   // wave-uniform
+    const int kk = min(32 * ks + 8 * kh, Np - 8);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) Ra[set][b] = *reinterpret_cast<const bf16x8*>(base + roffs[b] + kk);
+  };
+  auto loadT = [&](int ks, int set) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        Ta[set][c][b] = *reinterpret_cast<const bf16x8*>(Ts + (c * ROWS + 16 * b + cl) * LDK + 32 * ks + 8 * kh);
+  };
+  f32x4 X[NPH][2], Y[NPH][2];
+  auto zero = [&]() {
+#pragma unroll
+    for (int h = 0; h < NPH; ++h)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) X[h][c] = Y[h][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  };
+  // the block pairs (I <= J) in order p = 0 .. NPAIR-1; this wave takes p = 2 h + Q
+  auto mma_half = [&](auto Qc, int rs, int ts) {
+    constexpr int Q = decltype(Qc)::value;
+    int p = 0;
+#pragma unroll
+    for (int I = 0; I < NB; ++I)
+#pragma unroll
+      for (int J = I; J < NB; ++J, ++p)
+        if ((p & 1) == Q)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            X[p >> 1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ra[rs][I], Ta[ts][c][J], X[p >> 1][c], 0, 0, 0);
+            Y[p >> 1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ta[ts][c][I], Ra[rs][J], Y[p >> 1][c], 0, 0, 0);
+          }
+  };
+  auto pair_sum = [&](auto Qc, float& sr, float& si) {
+    constexpr int Q = decltype(Qc)::value;
+    int p = 0;
+#pragma unroll
+    for (int I = 0; I < NB; ++I)
+#pragma unroll
+      for (int J = I; J < NB; ++J, ++p)
+        if ((p & 1) == Q) {
+          float ar = 0.f, ai = 0.f;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            ar += X[p >> 1][0][r] * Y[p >> 1][0][r] - X[p >> 1][1][r] * Y[p >> 1][1][r];
+            ai += X[p >> 1][0][r] * Y[p >> 1][1][r] + X[p >> 1][1][r] * Y[p >> 1][0][r];
+          }
+          const float m = I == J ? 1.f : 2.f;
+          sr += m * ar;
+          si += m * ai;
+        }
+  };
+  // the whole stage loop for one half of the block pairs (one copy per half, so that
+  // neither loop branches on the half)
+  auto run = [&](auto Qc) {
+    double pr = 0.0, pi = 0.0;   // this lane's share of the current g-tile
+    auto finish_g = [&](int gi) {
+      if (gi >= ng) return;   // a padding stage
+      float sr = 0.f, si = 0.f;
+      pair_sum(Qc, sr, si);
+      const int tw = gi / gt, g = (t0 + u + PW_STREAMS * tw) * gt + gi % gt;
+      if (g < GE) {
+        pr += (double)sr;
+        pi += (double)si;
+      }
+      if (gi % gt == gt - 1) {   // last g of the tile
+        const double a = wave_sum(pr), b = wave_sum(pi);
+        if (lane == 0) {
+          part[q][u + PW_STREAMS * tw][0] = a;
+          part[q][u + PW_STREAMS * tw][1] = b;
+        }
+        pr = pi = 0.0;
+      }
+      zero();
+    };
+    // stages (gi, ks) in order, stage s in register set s & 1; an odd count gets a
+    // padding stage (its loads re-read the last g, its products are never used), so the
+    // loop has no branch around loads
+    const int nst = ng * nks;
+    zero();
+    int gR = 0, kR = 0;   // the stage whose fragments are requested next
+    auto advance = [&](int& g, int& k) {
+      if (++k == nks) { k = 0; ++g; }
+    };
+    if (nst > 0) {
+      loadR(g_of(0), 0, 0);
+      advance(gR, kR);
+      loadT(0, 0);
+    }
+    int gi = 0, ks = 0;
+    for (int s = 0; s < nst; s += 2) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        loadR(g_of(min(gR, ng - 1)), kR, h ^ 1);
+        loadT(kR, h ^ 1);
+        advance(gR, kR);
+        mma_half(Qc, h, h);
+        if (ks == nks - 1) finish_g(gi);
+        advance(gi, ks);
+      }
+    }
+  };
+  if (q == 0) run(std::integral_constant<int, 0>{});
+  else run(std::integral_constant<int, 1>{});
+  __syncthreads();
+  double* o = Xp + 2 * (((int64_t)zs * W + w) * ngt_cap);
+  for (int t = t0 + tid; t < t1; t += 64 * PW_WAVES) {
+    o[2 * t] = part[0][t - t0][0] + part[1][t - t0][0];
+    o[2 * t + 1] = part[0][t - t0][1] + part[1][t - t0][1];
+  }
+}
+
 static void launch_exchange_q8(const DevState& s, const int* sel, int nsel, hipStream_t st);
+// k_exchange_pw for the one-pass surrogate where it applies (slots of at most 64
+// electrons, N padded to 32 at most PW_KMAX); false: use k_exchange_bf16
+static bool exchange_pw(const DevState& s, const int* sel, int nsel, hipStream_t st) {
+  const int Kp = (s.Np + 31) & ~31;
+  int nmax = 0;
+  for (int z = 0; z < s.nslots; ++z) nmax = std::max(nmax, s.slot[z].n);
+  if (Kp > PW_KMAX || nmax > 64 || nsel <= 0) return false;
+  auto go = [&](int z0, int nz, const u16* rh, const u16* th, double* xp, Slot a, Slot b) {
+    const int nb = (a.n + 15) / 16;
+    const int gt = exchange_tile_g(a.n), ngt = (int)cdivu(s.GE, gt);
+    const int tpc = std::min(PW_TMAX, (int)cdivu(ngt, 4)), nchunk = (int)cdivu(ngt, tpc);
+    const dim3 grid((unsigned)(nsel * nz * nchunk)), blk(64 * PW_WAVES);
+#define AFQ_PW(NB_)                                                                                            \
+  hipLaunchKernelGGL(k_exchange_pw<NB_>, grid, blk, 0, st, rh, th, sel, xp, s.Np, s.GE, nsel, s.W, s.G, a, b, \
+                     s.ctot, s.ngt_cap, gt, tpc, nz, nchunk)
+    switch (nb) {
+      case 1: AFQ_PW(1); break;
+      case 2: AFQ_PW(2); break;
+      case 3: AFQ_PW(3); break;
+      default: AFQ_PW(4); break;
+    }
+#undef AFQ_PW
+    (void)z0;
+  };
+  if (s.nslots == 1 || s.slot[1].n == s.slot[0].n) {
+    go(0, s.nslots, s.Rb_hi, s.Tb_hi, s.Xp, s.slot[0], s.nslots > 1 ? s.slot[1] : s.slot[0]);
+  } else {
+    go(0, 1, s.Rb_hi, s.Tb_hi, s.Xp, s.slot[0], s.slot[0]);
+    Slot bb = s.slot[1];
+    bb.koff = s.slot[0].koff;   // koff sentinel: offsets zero, pointers advanced instead
+    go(1, 1, s.Rb_hi + (int64_t)s.G * s.slot[0].n * s.Np, s.Tb_hi + (int64_t)s.slot[0].n * s.Np,
+       s.Xp + 2 * (int64_t)s.W * s.ngt_cap, bb, bb);
+  }
+  return true;
+}
+
 void launch_exchange_lp(const DevState& s, const int* sel, int nsel, int prec, hipStream_t st) {
   if (prec == AFQ_PREC_FP8) {
     launch_exchange_q8(s, sel, nsel, st);
     return;
   }
   if (!s.tb_live) launch_theta_split(s, sel, nsel, st);   // else split by the force bias already
+  if (prec == AFQ_PREC_BF16 && s.xc_variant != 1 && exchange_pw(s, sel, nsel, st)) return;
   auto go = [&](int z0, int nz, double* xp) {
     const Slot& a = s.slot[z0];
     const int gt = exchange_tile_g(a.n), wt = exchange_tile_w(a.n);
