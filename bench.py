@@ -165,9 +165,17 @@ def stage_rooflines(spec, path, n_exact):
     elif path.get("surrogate_prec"):
         passes = 3 if path["surrogate_prec"] == "bf16x3" else 1
         nl = W * live
-        add("exchange_lp", "k_exchange_bf16 (%d-pass BF16 MFMA, surrogate level)" % passes,
-            exchange_flops(spec, nl), 2.0 * (2 if passes == 3 else 1) * (G * ncol * Np + 2 * ncol * Np * nl),
-            1, "bf16", symbol="void k_exchange_bf16<%d>" % (2 if passes == 3 else 1), passes=passes, walkers=nl)
+        nmax = max(nsp)
+        if passes == 1 and nmax <= 64 and (Np + 31) // 32 * 32 <= 512:
+            # the pair-wise kernel: theta of a walker in LDS, f_ij f_ji in registers
+            kname = "k_exchange_pw (1-pass BF16 MFMA, pair sums in registers, surrogate level)"
+            sym = "void k_exchange_pw<%d>" % (-(-nmax // 16))
+        else:
+            kname = "k_exchange_bf16 (%d-pass BF16 MFMA, surrogate level)" % passes
+            sym = "void k_exchange_bf16<%d>" % (2 if passes == 3 else 1)
+        add("exchange_lp", kname, exchange_flops(spec, nl),
+            2.0 * (2 if passes == 3 else 1) * (G * ncol * Np + 2 * ncol * Np * nl),
+            1, "bf16", symbol=sym, passes=passes, walkers=nl)
     # Taylor: V (both planes) once, t in, t out, the accumulator read and written, per walker and product
     tay_bytes = W * (16.0 * N * N + 4 * 16.0 * ms) * 5
     if path.get("fb_vhs_prec", "fp64") == "fp64":

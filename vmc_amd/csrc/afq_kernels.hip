@@ -2184,7 +2184,7 @@ __global__ __launch_bounds__(256, NS == 1 ? 4 : 2) void k_exchange_bf16(const u1
 // registers (off-diagonal pairs count twice, and the diagonal blocks are formed twice:
 // 25% more MFMA at NB = 4) with no LDS round trip.  Padded rows of theta are zero, so
 // the clamped (finite) R rows and K columns past the data contribute nothing.  The
-// loads have no branches and run one K stage ahead.  Eight waves, two per
+// loads have no branches and run two K stages ahead.  Eight waves, two per
 // SIMD: the two waves of a pair take the same g's and split the block pairs, so each
 // holds half the accumulators.  The per-g-tile partials go to LDS and leave at the end
 // in the layout of k_exchange_bf16's.
@@ -2230,8 +2230,8 @@ __global__ __launch_bounds__(64 * PW_WAVES, 1) void k_exchange_pw(const u16* __r
   int roffs[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) roffs[b] = min(16 * b + cl, n - 1) * Np;
-  // R and theta fragments run one stage ahead in two register sets
-  bf16x8 Ra[2][NB], Ta[2][2][NB];
+  // R and theta fragments run two stages ahead in three register sets
+  bf16x8 Ra[3][NB], Ta[3][2][NB];
   auto loadR = [&](int g, int ks, int set) {
     const u16* base = Rs + (int64_t)g * n * Np;   // wave-uniform
     const int kk = min(32 * ks + 8 * kh, Np - 8);
@@ -2309,9 +2309,9 @@ __global__ __launch_bounds__(64 * PW_WAVES, 1) void k_exchange_pw(const u16* __r
       }
       zero();
     };
-    // stages (gi, ks) in order, stage s in register set s & 1; an odd count gets a
-    // padding stage (its loads re-read the last g, its products are never used), so the
-    // loop has no branch around loads
+    // stages (gi, ks) in order, stage s in register set s % 3; the count is rounded up to
+    // a multiple of 3 with padding stages (their loads re-read the last g, their products
+    // are never used), so the loop has no branch around loads
     const int nst = ng * nks;
     zero();
     int gR = 0, kR = 0;   // the stage whose fragments are requested next
@@ -2319,16 +2319,19 @@ __global__ __launch_bounds__(64 * PW_WAVES, 1) void k_exchange_pw(const u16* __r
       if (++k == nks) { k = 0; ++g; }
     };
     if (nst > 0) {
-      loadR(g_of(0), 0, 0);
-      advance(gR, kR);
-      loadT(0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        loadR(g_of(min(gR, ng - 1)), kR, i);
+        loadT(kR, i);
+        advance(gR, kR);
+      }
     }
     int gi = 0, ks = 0;
-    for (int s = 0; s < nst; s += 2) {
+    for (int s = 0; s < nst; s += 3) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        loadR(g_of(min(gR, ng - 1)), kR, h ^ 1);
-        loadT(kR, h ^ 1);
+      for (int h = 0; h < 3; ++h) {
+        loadR(g_of(min(gR, ng - 1)), kR, (h + 2) % 3);
+        loadT(kR, (h + 2) % 3);
         advance(gR, kR);
         mma_half(Qc, h, h);
         if (ks == nks - 1) finish_g(gi);
