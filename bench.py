@@ -150,8 +150,13 @@ def stage_rooflines(spec, path, n_exact):
 
     # exchange: the rotated Cholesky once per launch + theta of each walker measured
     if path["exchange_prec"] == "fp64":
+        extra = {}
+        if path.get("surrogate_prec"):
+            # the exact subset of the control variate; the HBM counter summary keeps its
+            # dispatches apart from the unclipped shadow measurements on the same kernel
+            extra["profile_row"] = "k_exchange (exact subset)"
         add("exchange", "k_exchange (FP64 MFMA)", exchange_flops(spec, n_exact),
-            8.0 * G * ms + 16.0 * ms * n_exact, 1, "fp64", symbol="k_exchange", walkers=n_exact)
+            8.0 * G * ms + 16.0 * ms * n_exact, 1, "fp64", symbol="k_exchange", walkers=n_exact, **extra)
     else:
         add("exchange", "k_exchange_bf16 (3-pass split BF16 MFMA)", exchange_flops(spec, n_exact),
             4.0 * G * ncol * Np + 8.0 * ncol * Np * n_exact, 1, "bf16", symbol="void k_exchange_bf16<2>",
@@ -522,7 +527,7 @@ def main():
     dom_name = max(shares, key=shares.get)
     dom = dict(best["rooflines"][dom_name])
     dom["stage"] = dom_name
-    dom["traffic"] = profiled_traffic(a.workload, dom.get("symbol"))
+    dom["traffic"] = profiled_traffic(a.workload, dom.get("profile_row", dom.get("symbol")))
     if dom["traffic"] is not None:
         dom["traffic_source"] = os.path.relpath(PROFILE_HBM, ROOT)
     cpu = None
