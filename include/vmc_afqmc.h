@@ -38,6 +38,11 @@
  *                       DQMC/RHF.cpp:42-61; DQMC/UHF.cpp:45-65)
  *   afq_overlap         DQMCWalker::overlap (DQMC/DQMCWalker.cpp:559-566)
  *   afq_comb            reconfiguration (DQMC/MixedEstimator.cpp:559-635)
+ *   afq_rdm_accumulate  DQMCWalker::oneRDM loop of the mixed-estimator 1-RDM
+ *                       (DQMC/MixedEstimator.cpp:492-510; DQMC/DQMCWalker.cpp:579-589;
+ *                       DQMC/RHF.cpp:64-67; DQMC/UHF.cpp:68-72)
+ *   afq_get_one_rdm     oneRDM / cumulativeWeight as writeOneRDM writes them
+ *                       (DQMC/MixedEstimator.cpp:52-112)
  *   afq_run_long_prop   calcMixedEstimatorLongProp (DQMC/MixedEstimator.cpp:309-699)
  */
 #ifndef VMC_AFQMC_H
@@ -196,6 +201,20 @@ int afq_overlap(afq_ctx* ctx, double* overlaps);
  * weight to sum/W.  total_weight (nullable) receives sum. */
 int afq_comb(afq_ctx* ctx, double zeta, double* total_weight);
 
+/* Mixed-estimator one-body density matrix (DQMC/MixedEstimator.cpp:492-510).
+ * afq_rdm_accumulate adds one measurement: every walker of nonzero weight w
+ * contributes w Re(G_s^T) per spin slot s, G_s = psi_s (Phi_s^T psi_s)^-1 Phi_s^T
+ * (RHF::oneRDM DQMC/RHF.cpp:64-67, UHF::oneRDM DQMC/UHF.cpp:68-72), from the
+ * Green's function of the current determinants, and w to the cumulative weight.
+ * afq_get_one_rdm returns the weighted average (the reference keeps it as a
+ * running average, oneRDM = (oneRDM cw + w sample) / (cw + w)) and the cumulative
+ * weight: spin_summed = 1 gives one N x N row-major matrix, both spins added
+ * (RHF::oneRDM); spin_summed = 0 gives two, up then down (UHF::oneRDM).  With
+ * no accumulation yet both are zero.  afq_rdm_reset clears the accumulator. */
+int afq_rdm_reset(afq_ctx* ctx);
+int afq_rdm_accumulate(afq_ctx* ctx);
+int afq_get_one_rdm(afq_ctx* ctx, int32_t spin_summed, double* rdm, double* cumulative_weight);
+
 /* Per-stage device time (ms) accumulated since the last reset, AFQ_NSTAGES
  * entries in the order force_bias, fields, vhs, one-body (E psi), green, weight,
  * ortho, exchange, comb, taylor (the five V t products), exchange_lp (the
@@ -346,6 +365,11 @@ typedef struct afq_run_params {
   int32_t surrogate_kind; /* AFQ_SURR_MLP (trained first) | AFQ_SURR_EXCHANGE (no training) */
   int32_t surrogate_prec; /* precision of the AFQ_SURR_EXCHANGE level                      */
   int32_t prec_prop;      /* Taylor products (afq_set_prop_precision)                       */
+  double* one_rdm_out;    /* non-null: receives the mixed-estimator 1-RDM, accumulated at every
+                           * measurement with step dt > 10 and step > burn_iter nsteps
+                           * (DQMC/MixedEstimator.cpp:492-510): N x N spin-summed for an RHF
+                           * trial (RHF::oneRDM), 2 x N x N up then down for a UHF trial
+                           * (UHF::oneRDM); its cumulative weight is one_rdm_weight          */
 } afq_run_params;
 
 typedef struct afq_run_result {
@@ -371,6 +395,7 @@ typedef struct afq_run_result {
   double live_frac;            /* mean fraction of live walkers (weight > 0) over the timed steps */
   double shadow_clip_frac;     /* mean fraction of the live walkers whose exact E_L lies outside the
                                   clip window (running average +- sqrt(2/dt)), over the shadow blocks */
+  double one_rdm_weight;       /* cumulativeWeight of the 1-RDM (0 when none was accumulated) */
 } afq_run_result;
 
 /* ref_up/ref_dn: reference determinant (rhf.txt or uhf.txt by walker type,

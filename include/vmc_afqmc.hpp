@@ -119,7 +119,8 @@ class DQMCWalkers {
   // rhfQ: the walkers carry one determinant for both spins (DQMCWalker::rhfQ).
   DQMCWalkers(const Hamiltonian& ham, const Wavefunction& left, int nwalk, bool rhfQ, int device = 0,
               double weightCap = 0.0)
-      : nwalk_(nwalk), matSize_((size_t)ham.norbs * (ham.nalpha + ham.nbeta)), nchol_(ham.nchol) {
+      : nwalk_(nwalk), matSize_((size_t)ham.norbs * (ham.nalpha + ham.nbeta)), nchol_(ham.nchol),
+        norb_((size_t)ham.norbs) {
     afq_ham h = ham.c_struct();
     afq_trial t = left.c_struct();
     afq_params p{};
@@ -214,12 +215,22 @@ class DQMCWalkers {
     check(afq_comb(ctx_, zeta, &tot), ctx_, "afq_comb");
     return tot;
   }
+  // DQMC/MixedEstimator.cpp:492-510 (DQMCWalker::oneRDM of every live walker) for one
+  // measurement; oneRDM() returns the weighted average, N x N row-major per block
+  // (spinSummed: one block, RHF::oneRDM; else up then down, UHF::oneRDM)
+  void accumulateOneRDM() { check(afq_rdm_accumulate(ctx_), ctx_, "afq_rdm_accumulate"); }
+  std::vector<double> oneRDM(bool spinSummed, double* cumulativeWeight = nullptr) {
+    std::vector<double> r((spinSummed ? 1 : 2) * norb_ * norb_);
+    check(afq_get_one_rdm(ctx_, spinSummed ? 1 : 0, r.data(), cumulativeWeight), ctx_, "afq_get_one_rdm");
+    return r;
+  }
 
  private:
   afq_ctx* ctx_ = nullptr;
   int nwalk_;
   size_t matSize_;
   int nchol_;
+  size_t norb_;
 };
 
 // DQMC/MixedEstimator.cpp:309-699 on one GPU.  Returns the per-block

@@ -761,6 +761,46 @@ void orc_green_walker(orc_ctx* c, int w, int slot, double* theta, double* green)
       }
 }
 
+// One walker's term of the mixed-estimator 1-RDM (DQMC/MixedEstimator.cpp:492-510):
+// rdm = (rdm cw + w Re(sample)) / (cw + w), cw += w, where sample is
+// RHF::oneRDM's (psi_s (Phi^T psi_s)^-1 Phi^T)^T summed over the spins
+// (DQMC/RHF.cpp:64-67; spin_summed, one N x N block) or UHF::oneRDM's per-spin
+// pair (DQMC/UHF.cpp:68-72; two blocks, up then down).  rdm is row-major.
+static void one_rdm_walker(const orc_ctx* c, const Walker& W, double w, int spin_summed, double* rdm,
+                           double* cw) {
+  const int N = c->N;
+  const size_t NN = (size_t)N * N;
+  const int nblk = spin_summed ? 1 : 2;
+  std::vector<double> sample(nblk * NN, 0.0);
+  for (size_t q = 0; q < c->slots.size(); ++q) {  // slot 0: up, slot 1: down
+    const Slot& s = c->slots[q];
+    MC th;
+    green_slot(c, W, s, &th, nullptr);
+    MC G(N, N);  // psi (Phi^T psi)^-1 Phi^T
+    for (int j = 0; j < N; ++j)
+      for (int k = 0; k < s.n; ++k) {
+        const double pj = c->phi[s.phi](j, k);
+        for (int i = 0; i < N; ++i) G(i, j) += th(i, k) * pj;
+      }
+    for (int b = 0; b < nblk; ++b) {
+      // an RHF walker against an RHF trial: one slot of multiplicity 2 for both spins
+      const double m = spin_summed ? s.mult : (((int)q == b || s.mult == 2) ? 1.0 : 0.0);
+      if (m == 0.0) continue;
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) sample[b * NN + (size_t)i * N + j] += m * G(j, i).real();
+    }
+  }
+  for (size_t q = 0; q < nblk * NN; ++q) rdm[q] *= *cw;
+  for (size_t q = 0; q < nblk * NN; ++q) rdm[q] += w * sample[q];
+  *cw += w;
+  for (size_t q = 0; q < nblk * NN; ++q) rdm[q] /= *cw;
+}
+
+void orc_one_rdm(orc_ctx* c, int spin_summed, double* rdm, double* cw) {
+  for (size_t w = 0; w < c->walkers.size(); ++w)
+    if (c->weights[w] != 0.) one_rdm_walker(c, c->walkers[w], c->weights[w], spin_summed, rdm, cw);
+}
+
 void orc_ham_and_overlap_dets(orc_ctx* c, const double* dets, int ndet, int nchol_ene,
                               double* out) {
   const size_t ms = (size_t)c->N * (c->na + c->nb);
@@ -900,6 +940,10 @@ int orc_run_long_prop(const afq_ham* ham, const afq_trial* left, const double* r
   const double weightCap = p->weight_cap > 0 ? p->weight_cap : std::max(100., nw / 10.);
   std::vector<double> x(G);
   double propTime = 0, eneTime = 0;
+  // mixed-estimator 1-RDM (:440-450, 492-510): RHF trial spin-summed, UHF trial per spin
+  const int rdm_summed = c->trial_rhf ? 1 : 0;
+  double rdm_cw = 0.0;
+  if (p->one_rdm_out) std::fill(p->one_rdm_out, p->one_rdm_out + (rdm_summed ? 1 : 2) * (size_t)N * N, 0.0);
   const int64_t last = (int64_t)nsweeps * nsteps;
   for (int64_t step = 1; step < last; ++step) {
     if (p->max_steps > 0 && step > p->max_steps) break;
@@ -930,6 +974,8 @@ int orc_run_long_prop(const afq_ham* ham, const afq_trial* left, const double* r
         for (int wl = 0; wl < nw; ++wl) {
           const int w = r * nw + wl;
           if (c->weights[w] != 0.) {
+            if (p->one_rdm_out && step * dt > 10. && step > (int64_t)p->burn_iter * nsteps)
+              one_rdm_walker(c, c->walkers[w], c->weights[w], rdm_summed, p->one_rdm_out, &rdm_cw);
             cd o[2];
             ham_and_overlap(c, c->walkers[w], c->GE, o);
             eloc[w] = (o[0] / o[1]).real() + delta.real();
@@ -979,6 +1025,7 @@ int orc_run_long_prop(const afq_ham* ham, const afq_trial* left, const double* r
     res->n_large_deviations = nLarge;
     res->prop_time = propTime;
     res->ene_time = eneTime;
+    res->one_rdm_weight = rdm_cw;
   }
   delete c;
   return AFQ_OK;

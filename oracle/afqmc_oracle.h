@@ -61,6 +61,11 @@ void orc_green_walker(orc_ctx* c, int w, int slot, double* theta, double* green)
  * the one-body part, for walker w (complex each). */
 void orc_energy_parts_walker(orc_ctx* c, int w, double* e1, double* ej, double* ek);
 void orc_comb(orc_ctx* c, double zeta, double* total_weight);
+/* One measurement of the mixed-estimator 1-RDM (DQMC/MixedEstimator.cpp:492-510)
+ * over the walkers of nonzero weight, in order, as the reference's running
+ * average: rdm (row-major; spin_summed: one N x N block, RHF::oneRDM; else two,
+ * up then down, UHF::oneRDM) and cw (cumulative weight) carry the state. */
+void orc_one_rdm(orc_ctx* c, int spin_summed, double* rdm, double* cw);
 
 /* Driver: calcMixedEstimatorLongProp with nranks virtual MPI ranks of nwalk
  * walkers each, rank r drawing from std::mt19937(seed + r). */
@@ -79,6 +84,8 @@ typedef struct orc_run_params {
   double ene0_guess;       /* 1e10: use the reference energy */
   double weight_cap;       /* <= 0: default                  */
   int32_t max_steps;       /* > 0: stop after this many steps (partial run) */
+  double* one_rdm_out;     /* non-null: the mixed-estimator 1-RDM (RHF trial: N x N
+                            * spin-summed; UHF trial: up, down) */
 } orc_run_params;
 
 typedef struct orc_run_result {
@@ -87,6 +94,7 @@ typedef struct orc_run_result {
   double delta;
   int64_t n_large_deviations;
   double prop_time, ene_time;
+  double one_rdm_weight;   /* cumulativeWeight of the 1-RDM */
 } orc_run_result;
 
 /* ref_up/ref_dn: reference dets (rhf.txt or uhf.txt per walker type);

@@ -23,13 +23,14 @@ class RunParams(C.Structure):
         ("nwalk", C.c_int32), ("nranks", C.c_int32), ("walker_rhf", C.c_int32),
         ("ref_compat", C.c_int32), ("cholesky_threshold", C.c_double),
         ("ene0_guess", C.c_double), ("weight_cap", C.c_double), ("max_steps", C.c_int32),
+        ("one_rdm_out", DP),
     ]
 
 
 class RunResult(C.Structure):
     _fields_ = [("ref_energy", C.c_double), ("nchol_ene", C.c_int32), ("delta", C.c_double),
                 ("n_large_deviations", C.c_int64), ("prop_time", C.c_double),
-                ("ene_time", C.c_double)]
+                ("ene_time", C.c_double), ("one_rdm_weight", C.c_double)]
 
 
 _lib = None
@@ -62,6 +63,7 @@ def lib():
         L.orc_energy_parts_walker.argtypes = [vp, C.c_int, DP, DP, DP]
         L.orc_ham_and_overlap_dets.argtypes = [vp, DP, C.c_int, C.c_int, DP]
         L.orc_comb.argtypes = [vp, C.c_double, DP]
+        L.orc_one_rdm.argtypes = [vp, C.c_int, DP, DP]
         L.orc_run_long_prop.restype = C.c_int
         L.orc_run_long_prop.argtypes = [C.POINTER(AfqHam), C.POINTER(AfqTrial), DP, DP, DP, DP,
                                         C.POINTER(RunParams), DP, DP, C.POINTER(RunResult)]
@@ -187,6 +189,16 @@ class Oracle:
         lib().orc_comb(self.ptr, zeta, dptr(t))
         return t[0]
 
+    def one_rdm(self, spin_summed, rdm=None, cw=0.0):
+        """One measurement of the mixed-estimator 1-RDM, continuing the running
+        average (rdm, cw); returns the new (rdm, cw)."""
+        N = self.h.norb
+        shape = (N, N) if spin_summed else (2, N, N)
+        rdm = np.zeros(shape) if rdm is None else np.array(rdm, dtype=np.float64).reshape(shape)
+        c = np.array([cw], dtype=np.float64)
+        lib().orc_one_rdm(self.ptr, int(spin_summed), dptr(rdm), dptr(c))
+        return rdm, c[0]
+
     def bench(self, nthreads, nsteps, energy_every, ortho_steps, seed):
         s = np.zeros(1)
         r = lib().orc_bench(self.ptr, nthreads, nsteps, energy_every, ortho_steps, seed, dptr(s))
@@ -196,12 +208,14 @@ class Oracle:
 def run_long_prop(case_ham, left_is_rhf, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps,
                   nsweeps, ortho_steps, nwalk, nranks, walker_rhf, ref_compat=False,
                   cholesky_threshold=1e-3, burn_iter=0, ene0_guess=1e10, weight_cap=0.0,
-                  max_steps=0):
+                  max_steps=0, one_rdm=None):
+    """one_rdm: None, or an array the mixed-estimator 1-RDM is written into (N x N for
+    an RHF trial, 2 x N x N for a UHF trial)."""
     hs = case_ham.ham_struct()
     ts = case_ham.trial_struct()
     p = RunParams(seed, dt, nsteps, nsweeps, ortho_steps, burn_iter, nwalk, nranks,
                   int(walker_rhf), int(ref_compat), cholesky_threshold, ene0_guess,
-                  weight_cap, max_steps)
+                  weight_cap, max_steps, dptr(one_rdm))
     w = np.zeros(nsweeps)
     e = np.zeros(nsweeps)
     res = RunResult()

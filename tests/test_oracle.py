@@ -298,3 +298,73 @@ def test_comb_matches_searchsorted():
     assert np.allclose(w2, cum[-1] / nw)
     assert np.array_equal(d2, dets[idx])
     assert 3 not in idx
+
+
+# ---------------------------------------------------------------- mixed-estimator 1-RDM
+
+@pytest.mark.parametrize("trial,walker_rhf", [("uhf", False), ("rhf", True), ("rhf", False), ("uhf", True)])
+def test_one_rdm_invariants(trial, walker_rhf):
+    """Each sample Re(G_s^T), G_s = Theta_s Phi_s^T, has trace n_s and fixes Phi_s
+    (G_s^T Phi_s = Phi_s Theta_s^T Phi_s = Phi_s since Phi_s^T Theta_s = I); a weighted
+    average keeps both.  A transposed sample (G_s Phi_s = Theta_s) or a dropped
+    multiplicity fails them.  Walkers of weight zero do not count."""
+    rng = np.random.default_rng(31)
+    h = small_system(trial, na=3, nb=3 if (walker_rhf or trial == "rhf") else 2)
+    nw = 5
+    orc = O.Oracle(h, nw, walker_rhf)
+    dets = random_walkers(h, nw, rng)
+    if walker_rhf:
+        for w in range(nw):
+            up, _ = h.split_det(dets[w])
+            dets[w] = h.serial_det(up, up)
+    wts = rng.uniform(0.2, 2.0, nw)
+    wts[2] = 0.0
+    orc.set_walkers(dets, weights=wts)
+    per, cw = orc.one_rdm(0)
+    assert abs(cw - wts.sum()) < 1e-14 * wts.sum()
+    for s, (phi, n) in enumerate([(h.phi_up, h.nalpha), (h.phi_dn, h.nbeta)]):
+        assert abs(np.trace(per[s]) - n) < 1e-12
+        assert np.allclose(per[s] @ phi, phi, atol=1e-12)
+        assert not np.allclose(per[s], per[s].T, atol=1e-6)  # the sample is not symmetric here
+    tot, _ = orc.one_rdm(1)
+    assert abs(np.trace(tot) - (h.nalpha + h.nbeta)) < 1e-12
+    assert np.allclose(tot, per[0] + per[1], atol=1e-13)
+    if trial == "rhf":
+        assert np.allclose(tot @ h.phi_up, 2 * h.phi_up, atol=1e-12)
+
+
+def test_one_rdm_at_trial_is_projector():
+    """psi = Phi (orthonormal): Theta = Phi and every sample is Phi Phi^T."""
+    h = small_system("uhf")
+    orc = O.Oracle(h, 2, False)
+    orc.set_walkers(np.array([h.serial_det(h.phi_up, h.phi_dn)] * 2), weights=np.array([0.5, 1.5]))
+    per, cw = orc.one_rdm(0)
+    assert cw == 2.0
+    assert np.allclose(per[0], h.phi_up @ h.phi_up.T, atol=1e-13)
+    assert np.allclose(per[1], h.phi_dn @ h.phi_dn.T, atol=1e-13)
+
+
+def test_one_rdm_running_average_over_measurements():
+    """Two measurements continue one running average: the result is the weighted
+    mean of the samples of both populations (the reference's update
+    rdm = (rdm cw + w s) / (cw + w), DQMC/MixedEstimator.cpp:495-498, equals it)."""
+    rng = np.random.default_rng(32)
+    h = small_system("uhf")
+    nw = 4
+    orc = O.Oracle(h, nw, False)
+    samples, weights = [], []
+    state = (None, 0.0)
+    for m in range(2):
+        dets = random_walkers(h, nw, rng)
+        wts = rng.uniform(0.1, 3.0, nw)
+        orc.set_walkers(dets, weights=wts)
+        state = orc.one_rdm(0, *state)
+        for w in range(nw):
+            up, dn = h.split_det(dets[w])
+            s = [(psi @ np.linalg.solve(phi.T @ psi, phi.T)).T.real
+                 for psi, phi in ((up, h.phi_up), (dn, h.phi_dn))]
+            samples.append(np.array(s))
+            weights.append(wts[w])
+    ref = np.tensordot(np.array(weights), np.array(samples), axes=1) / np.sum(weights)
+    assert abs(state[1] - np.sum(weights)) < 1e-13 * np.sum(weights)
+    assert np.allclose(state[0], ref, rtol=0, atol=1e-13)

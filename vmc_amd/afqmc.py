@@ -49,6 +49,9 @@ def lib():
         L.afq_force_bias.argtypes = [vp, DP]
         L.afq_overlap.argtypes = [vp, DP]
         L.afq_comb.argtypes = [vp, d, DP]
+        L.afq_rdm_reset.argtypes = [vp]
+        L.afq_rdm_accumulate.argtypes = [vp]
+        L.afq_get_one_rdm.argtypes = [vp, i32, DP, DP]
         L.afq_stage_times.argtypes = [vp, DP, i32]
         L.afq_sync.argtypes = [vp]
         L.afq_debug_gemm_f64.argtypes = [DP, DP, DP, i32, i32, i32]
@@ -240,6 +243,21 @@ class Context:
         self._chk(lib().afq_comb(self.ptr, zeta, dptr(t)), "afq_comb")
         return t[0]
 
+    def rdm_reset(self):
+        self._chk(lib().afq_rdm_reset(self.ptr), "afq_rdm_reset")
+
+    def rdm_accumulate(self):
+        """One measurement of the mixed-estimator 1-RDM (every walker of nonzero weight)."""
+        self._chk(lib().afq_rdm_accumulate(self.ptr), "afq_rdm_accumulate")
+
+    def one_rdm(self, spin_summed):
+        """(weighted-average 1-RDM, cumulative weight): N x N if spin_summed, else 2 x N x N."""
+        N = self.h.norb
+        r = np.zeros((N, N) if spin_summed else (2, N, N))
+        cw = np.zeros(1)
+        self._chk(lib().afq_get_one_rdm(self.ptr, int(spin_summed), dptr(r), dptr(cw)), "afq_get_one_rdm")
+        return r, cw[0]
+
     def stage_times(self, reset=True):
         ms = np.zeros(len(STAGES))
         self._chk(lib().afq_stage_times(self.ptr, dptr(ms), int(reset)), "afq_stage_times")
@@ -421,7 +439,7 @@ class RunParams(C.Structure):
         ("surrogate_train_steps", C.c_int32), ("surrogate_hidden", C.c_int32),
         ("surrogate_epochs", C.c_int32), ("subset_inv", C.c_int32), ("fallback_rms", C.c_double),
         ("risk_frac", C.c_double), ("shadow_exact", C.c_int32), ("surrogate_kind", C.c_int32),
-        ("surrogate_prec", C.c_int32), ("prec_prop", C.c_int32),
+        ("surrogate_prec", C.c_int32), ("prec_prop", C.c_int32), ("one_rdm_out", DP),
     ]
 
 
@@ -433,7 +451,8 @@ class RunResult(C.Structure):
                 ("cv_var_sub", C.c_double), ("cv_rms", C.c_double), ("cv_fallbacks", C.c_int64),
                 ("train_samples", C.c_int64), ("est_var", C.c_double), ("shadow_n", C.c_int64),
                 ("shadow_shift", C.c_double), ("shadow_var", C.c_double), ("shadow_est_var", C.c_double),
-                ("shadow_wall_s", C.c_double), ("live_frac", C.c_double), ("shadow_clip_frac", C.c_double)]
+                ("shadow_wall_s", C.c_double), ("live_frac", C.c_double), ("shadow_clip_frac", C.c_double),
+                ("one_rdm_weight", C.c_double)]
 
 
 def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, nsweeps, ortho_steps,
@@ -443,8 +462,11 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   prec_vhs=AFQ_PREC_FP64, prec_exchange=AFQ_PREC_FP64,
                   energy_mode=AFQ_ENERGY_EXACT, device=0, surrogate_train_steps=0, surrogate_hidden=256,
                   surrogate_epochs=20, subset_inv=16, fallback_rms=0.0, risk_frac=0.05, shadow_exact=False,
-                  surrogate_kind=AFQ_SURR_MLP, surrogate_prec=AFQ_PREC_BF16, prec_prop=AFQ_PREC_FP64):
-    """calcMixedEstimatorLongProp on the GPU; returns (weights, energies, RunResult)."""
+                  surrogate_kind=AFQ_SURR_MLP, surrogate_prec=AFQ_PREC_BF16, prec_prop=AFQ_PREC_FP64,
+                  one_rdm=None):
+    """calcMixedEstimatorLongProp on the GPU; returns (weights, energies, RunResult).
+    one_rdm: None, or a float64 array that receives the mixed-estimator 1-RDM
+    (N x N for an RHF trial, 2 x N x N for a UHF trial)."""
     L = lib()
     if not hasattr(L, "_run_bound"):
         L.afq_run_long_prop.argtypes = [C.POINTER(AfqHam), C.POINTER(AfqTrial), DP, DP, DP, DP,
@@ -455,7 +477,7 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   int(ref_compat), cholesky_threshold, ene0_guess, weight_cap, max_steps, rng_mode,
                   warmup_steps, prec_fb, prec_vhs, prec_exchange, energy_mode, device,
                   surrogate_train_steps, surrogate_hidden, surrogate_epochs, subset_inv, fallback_rms,
-                  risk_frac, int(shadow_exact), surrogate_kind, surrogate_prec, prec_prop)
+                  risk_frac, int(shadow_exact), surrogate_kind, surrogate_prec, prec_prop, dptr(one_rdm))
     w = np.zeros(nsweeps)
     e = np.zeros(nsweeps)
     res = RunResult()

@@ -119,6 +119,9 @@ struct afq_ctx {
   float* cvX = nullptr;
   double cv_stats[6] = {0, 0, 0, 0, 0, 0};
   bool last_exact_all = false;
+  // mixed-estimator 1-RDM: acc[e] = sum w Re theta[e] (e < Ktot), acc[Ktot] = sum w
+  double* rdmAcc = nullptr;
+  double* rdmPart = nullptr;
 
   hipEvent_t ev_get() {
     hipEvent_t e;
@@ -630,6 +633,8 @@ int afq_destroy(afq_ctx* c) {
   free_all(c->s);
   free_mlp(c);
   free_cv(c);
+  if (c->rdmAcc) (void)hipFree(c->rdmAcc);
+  if (c->rdmPart) (void)hipFree(c->rdmPart);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return AFQ_OK;
@@ -1046,6 +1051,74 @@ int afq_comb(afq_ctx* c, double zeta, double* total_weight) {
     launch_check("comb");
   }
   if (total_weight) download(total_weight, c->s.cum + (c->s.W - 1), 1, c->stream);
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+// Mixed-estimator one-body density matrix (DQMC/MixedEstimator.cpp:492-510).
+int afq_rdm_reset(afq_ctx* c) {
+  if (!c) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  if (!c->rdmAcc) c->rdmAcc = dalloc<double>(s.Ktot + 1);
+  afq::launch_fill(c->rdmAcc, s.Ktot + 1, 0.0, c->stream);
+  launch_check("rdm reset");
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+int afq_rdm_accumulate(afq_ctx* c) {
+  if (!c) return AFQ_ERR_ARG;
+  if (!c->have_walkers) return fail(c, AFQ_ERR_STATE, "afq_set_walkers first");
+  if (!c->rdmAcc) {
+    const int rc = afq_rdm_reset(c);
+    if (rc) return rc;
+  }
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  if (!c->rdmPart) c->rdmPart = dalloc<double>(afq::rdm_part_doubles(s));
+  afq::launch_rdm_accumulate(s, c->rdmPart, c->rdmAcc, c->stream);
+  launch_check("rdm accumulate");
+  return AFQ_OK;
+  AFQ_GUARD_END(c)
+}
+
+// RDM_s(i, j) = sum_e Phi_s(i, e) acc_s(j, e) / sum w: the running average of
+// Re(theta_s Phi_s^T)^T.  spin_summed: the slots' matrices added with their
+// multiplicity (RHF::oneRDM, DQMC/RHF.cpp:64-67); otherwise [up | down]
+// (UHF::oneRDM, DQMC/UHF.cpp:68-72): slot 0 is the up spin and slot 1 the down
+// spin, and a single slot of multiplicity 2 stands for both.
+int afq_get_one_rdm(afq_ctx* c, int32_t spin_summed, double* rdm, double* cumulative_weight) {
+  if (!c || !rdm) return AFQ_ERR_ARG;
+  AFQ_GUARD_BEGIN
+  DevState& s = c->s;
+  const int N = s.N;
+  const size_t NN = (size_t)N * N;
+  std::fill(rdm, rdm + (spin_summed ? 1 : 2) * NN, 0.0);
+  std::vector<double> acc(s.Ktot + 1, 0.0);
+  if (c->rdmAcc) download(acc.data(), c->rdmAcc, s.Ktot + 1, c->stream);
+  const double cw = acc[s.Ktot];
+  if (cumulative_weight) *cumulative_weight = cw;
+  if (cw == 0.0) return AFQ_OK;
+  std::vector<double> M(NN);
+  for (int q = 0; q < s.nslots; ++q) {
+    const afq::Slot& sl = s.slot[q];
+    const double* ph = c->phi[sl.phi].data();  // N x n_s row-major
+    const double* a = acc.data() + sl.koff;    // [j][e]: orbital j, electron e
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        double v = 0.0;
+        for (int e = 0; e < sl.n; ++e) v += ph[(size_t)i * sl.n + e] * a[(size_t)j * sl.n + e];
+        M[(size_t)i * N + j] = v / cw;
+      }
+    if (spin_summed) {
+      for (size_t k = 0; k < NN; ++k) rdm[k] += sl.mult * M[k];
+    } else {
+      for (int d = 0; d < 2; ++d)
+        if (q == d || sl.mult == 2)
+          for (size_t k = 0; k < NN; ++k) rdm[d * NN + k] += M[k];
+    }
+  }
   return AFQ_OK;
   AFQ_GUARD_END(c)
 }

@@ -127,6 +127,7 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   double propTime = 0., eneTime = 0., tStart = 0.;
   int64_t timed = 0;
   afq_stage_times(c, nullptr, 1);
+  if (p->one_rdm_out && (rc = afq_rdm_reset(c))) return rc;
   const bool cv = p->energy_mode == AFQ_ENERGY_SURROGATE;
   const bool mlp = cv && p->surrogate_kind == AFQ_SURR_MLP;
   const int ntrain = mlp ? std::max(1, p->surrogate_train_steps) : 0;
@@ -217,6 +218,9 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     if (step % nsteps == 0) {
       const int block = (int)(step / nsteps);
       t0 = now_s();
+      // the 1-RDM of the walkers about to be measured (:492-510)
+      if (p->one_rdm_out && step * dt > 10. && step > (int64_t)p->burn_iter * nsteps)
+        if ((rc = afq_rdm_accumulate(c))) return rc;
       const bool exact_now = !cv || nmeasured < ntrain;
       // training pairs come from the FP64 path whatever the production precision
       if (cv && exact_now && (rc = afq_set_precision(c, AFQ_PREC_FP64, AFQ_PREC_FP64, AFQ_PREC_FP64))) return rc;
@@ -324,6 +328,8 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
   }
   afq_sync(c);
   const double tEnd = now_s();
+  double rdm_cw = 0.0;
+  if (p->one_rdm_out && (rc = afq_get_one_rdm(c, left->rhf ? 1 : 0, p->one_rdm_out, &rdm_cw))) return rc;
   for (int i = 0; i < nsweeps; ++i) {
     if (weights_out) weights_out[i] = totalWeights[i];
     if (energies_out) energies_out[i] = totalEnergies[i];
@@ -355,6 +361,7 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     res->shadow_wall_s = sh_wall;
     res->shadow_clip_frac = sh_n ? sh_clip / sh_n : 0.0;
     res->live_frac = timed ? live_sum / timed : 0.0;
+    res->one_rdm_weight = rdm_cw;
   }
   return AFQ_OK;
 }

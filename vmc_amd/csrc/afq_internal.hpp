@@ -176,6 +176,10 @@ void launch_qr(const DevState& s, hipStream_t st);
 void launch_exchange(const DevState& s, const int* sel, int nsel, hipStream_t st);
 void launch_energy(const DevState& s, const int* sel, int nsel, double delta, hipStream_t st);
 void launch_comb(DevState& s, double zeta, hipStream_t st);
+// 1-RDM accumulation: acc[e] += sum_w w Re theta[w][e] (e < Ktot), acc[Ktot] += sum_w w,
+// over the walkers of nonzero weight; part holds rdm_part_doubles(s) doubles
+int64_t rdm_part_doubles(const DevState& s);
+void launch_rdm_accumulate(const DevState& s, double* part, double* acc, hipStream_t st);
 inline int onebody_pitch(int N) { return (N + 31) / 32 * 32; }
 void launch_v_split(const DevState& s, hipStream_t st);
 void launch_rq8(const DevState& s, hipStream_t st);

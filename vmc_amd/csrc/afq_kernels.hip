@@ -2891,12 +2891,70 @@ void launch_energy(const DevState& s, const int* sel, int nsel, double delta, hi
                      ng1, s.ngt_cap, s.ecore, delta);
 }
 
+// ------------------------------------------------------------------ one-body density matrix
+// Mixed-estimator 1-RDM (DQMC/MixedEstimator.cpp:492-510): the weighted sum over
+// walkers of Re(G_s^T) = Phi_s Re(theta_s)^T needs only sum_w w Re(theta_s[w])
+// (Phi_s is real).  Element e < Ktot of a walker's theta carries that sum; element
+// Ktot carries sum_w w.  Pass 1: RDM_CH fixed chunks of walkers each add their
+// walkers of nonzero weight in ascending order into part[ch][e]; pass 2 adds the
+// chunks in order into acc[e], so the sums do not depend on scheduling.  Walkers
+// of weight zero are not read (their theta is not kept current).
+constexpr int RDM_CH = 32;
+__global__ __launch_bounds__(256) void k_rdm_part(const double* __restrict__ theta,
+                                                  const double* __restrict__ weights, int W, int64_t Ktot,
+                                                  double* __restrict__ part) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > Ktot) return;
+  const int per = (W + RDM_CH - 1) / RDM_CH;
+  const int w0 = blockIdx.y * per, w1 = min(W, w0 + per);
+  double s = 0.0;
+  for (int w = w0; w < w1; ++w) {
+    const double wt = weights[w];
+    if (wt == 0.0) continue;
+    s += wt * (e < Ktot ? theta[2 * ((int64_t)w * Ktot + e)] : 1.0);
+  }
+  part[(int64_t)blockIdx.y * (Ktot + 1) + e] = s;
+}
+__global__ __launch_bounds__(256) void k_rdm_add(const double* __restrict__ part, double* __restrict__ acc,
+                                                 int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = acc[e];
+  for (int ch = 0; ch < RDM_CH; ++ch) s += part[(int64_t)ch * n + e];
+  acc[e] = s;
+}
+int64_t rdm_part_doubles(const DevState& s) { return (int64_t)RDM_CH * (s.Ktot + 1); }
+void launch_rdm_accumulate(const DevState& s, double* part, double* acc, hipStream_t st) {
+  const int64_t n = s.Ktot + 1;
+  const unsigned bx = (unsigned)cdivu(n, 256);
+  hipLaunchKernelGGL(k_rdm_part, dim3(bx, RDM_CH), dim3(256), 0, st, s.theta, s.weights, s.W, s.Ktot, part);
+  hipLaunchKernelGGL(k_rdm_add, dim3(bx), dim3(256), 0, st, part, acc, n);
+}
+
 // ------------------------------------------------------------------ comb
-// sequential cumulative sum, exactly the reference's order (DQMC/MixedEstimator.cpp:595-598)
-__global__ void k_cumsum(const double* w, double* cum, int W) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double c = 0.0;
-    for (int i = 0; i < W; ++i) { c = (i == 0) ? w[0] : c + w[i]; cum[i] = c; }
+// Sequential cumulative sum, exactly the reference's order (DQMC/MixedEstimator.cpp:595-598).
+// The block stages chunks of the weights through LDS with coalesced loads; one thread
+// runs the dependent additions out of LDS, in order, carrying the sum across chunks.
+constexpr int CUMSUM_CHUNK = 4096;
+__global__ __launch_bounds__(256) void k_cumsum(const double* __restrict__ w, double* __restrict__ cum, int W) {
+  __shared__ double buf[CUMSUM_CHUNK];
+  double c = 0.0;  // the running sum, kept by thread 0
+  for (int base = 0; base < W; base += CUMSUM_CHUNK) {
+    const int n = min(CUMSUM_CHUNK, W - base);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) buf[i] = w[base + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int i = 0;
+      if (base == 0) { c = buf[0]; i = 1; }
+#pragma unroll 8
+      for (; i < n; ++i) {
+        c += buf[i];
+        buf[i] = c;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) cum[base + i] = buf[i];
+    __syncthreads();
   }
 }
 __global__ void k_comb_index(const double* cum, int* idx, double* weights, int W, double zeta,
@@ -2962,7 +3020,7 @@ void launch_step_inc(unsigned long long* step, hipStream_t st) {
 
 void launch_comb(DevState& s, double zeta, hipStream_t st) {
   s.tb_live = false;
-  hipLaunchKernelGGL(k_cumsum, dim3(1), dim3(64), 0, st, s.weights, s.cum, s.W);
+  hipLaunchKernelGGL(k_cumsum, dim3(1), dim3(256), 0, st, s.weights, s.cum, s.W);
   hipLaunchKernelGGL(k_comb_index, dim3(cdivu(s.W, 256)), dim3(256), 0, st, s.cum, s.idx, s.weights, s.W, zeta,
                      s.d_zeta);
   // eager calls swap the buffers; a captured step copies back so every replay sees
