@@ -81,3 +81,22 @@ def write_samples(path, weights, energies):
     with open(path, "w") as f:
         for w, e in zip(weights, energies):
             f.write("%.7e      %.10e \n" % (w, e))
+
+
+def write_one_rdm(scratch_dir, rdm, cumulative_weight, rank=0):
+    """writeOneRDM (DQMC/MixedEstimator.cpp:52-112): the cumulative weight on the
+    first line, then the matrix row by row, each entry followed by two spaces, in
+    the default ostream format (6 significant digits, C's %g).  A 2 x N x N rdm
+    (per spin) goes to rdm_up_<rank>.dat and rdm_dn_<rank>.dat, an N x N one to
+    rdm_<rank>.dat.  Returns the paths written."""
+    rdm = np.asarray(rdm, dtype=np.float64)
+    blocks = [("rdm_", rdm)] if rdm.ndim == 2 else [("rdm_up_", rdm[0]), ("rdm_dn_", rdm[1])]
+    paths = []
+    for prefix, m in blocks:
+        path = "%s/%s%d.dat" % (scratch_dir, prefix, rank)
+        with open(path, "w") as f:
+            f.write("%g\n" % cumulative_weight)
+            for row in m:
+                f.write("".join("%g  " % v for v in row) + "\n")
+        paths.append(path)
+    return paths
