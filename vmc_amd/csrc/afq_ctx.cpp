@@ -422,6 +422,7 @@ void refresh_walkers(afq_ctx* c, bool overlaps_from_dets) {
     afq::launch_green(s, true, c->stream);
     launch_check("green");
   }
+  s.theta_stale = false;
   if (overlaps_from_dets) {
     afq::launch_logO_from_logdet(s, c->stream);
     launch_check("logO");
@@ -429,6 +430,17 @@ void refresh_walkers(afq_ctx* c, bool overlaps_from_dets) {
   Stage t(c, afq::ST_FB);
   run_force_bias(s, c->stream);
   launch_check("force_bias");
+}
+
+// theta of every walker after an eager comb (which does not carry it), for the entry
+// points that read it before the next propagation
+void ensure_theta(afq_ctx* c) {
+  DevState& s = c->s;
+  if (!s.theta_stale) return;
+  Stage t(c, afq::ST_GREEN);
+  afq::launch_green(s, true, c->stream);
+  launch_check("green");
+  s.theta_stale = false;
 }
 
 // Indices of the live walkers (weight != 0), also uploaded to c->liveSel.  Dead
@@ -808,8 +820,10 @@ int afq_propagate(afq_ctx* c, const double* draws, uint64_t seed, uint64_t step,
   }
   {
     Stage t(c, afq::ST_GREEN);
-    afq::launch_green(s, false, c->stream);
+    // after a comb every walker's theta is recomputed, propagated or not
+    afq::launch_green(s, s.theta_stale, c->stream);
     launch_check("green");
+    s.theta_stale = false;
   }
   {
     Stage t(c, afq::ST_WEIGHT);
@@ -982,6 +996,7 @@ int afq_measure(afq_ctx* c, int32_t mode, double delta, double* eloc, double* ha
     return fail(c, AFQ_ERR_UNSUPPORTED, "energy mode not available");
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
+  ensure_theta(c);
   if (mode == AFQ_ENERGY_SURROGATE) {
     if (ham_ovlp) return fail(c, AFQ_ERR_ARG, "ham_ovlp is not produced by the surrogate mode");
     return measure_cv(c, delta, eloc);
@@ -1050,6 +1065,7 @@ int afq_comb(afq_ctx* c, double zeta, double* total_weight) {
     afq::launch_comb(c->s, zeta, c->stream);
     launch_check("comb");
   }
+  c->last_exact_all = false;  // the last measurement's per-walker partials refer to the old walkers
   if (total_weight) download(total_weight, c->s.cum + (c->s.W - 1), 1, c->stream);
   return AFQ_OK;
   AFQ_GUARD_END(c)
@@ -1077,6 +1093,7 @@ int afq_rdm_accumulate(afq_ctx* c) {
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
   if (!c->rdmPart) c->rdmPart = dalloc<double>(afq::rdm_part_doubles(s));
+  ensure_theta(c);
   afq::launch_rdm_accumulate(s, c->rdmPart, c->rdmAcc, c->stream);
   launch_check("rdm accumulate");
   return AFQ_OK;
@@ -1249,6 +1266,7 @@ int afq_surrogate_collect(afq_ctx* c, int64_t* total) {
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
   afq::Mlp& m = c->mlp;
+  ensure_theta(c);
   // the risk stratum is always evaluated exactly, so the surrogate learns the rest
   std::vector<char> lv, risky;
   risk_split(c, lv, risky);
@@ -1432,6 +1450,7 @@ int afq_surrogate_predict(afq_ctx* c, double* pred) {
   if (!c || !c->mlp.dims[0]) return c ? fail(c, AFQ_ERR_STATE, "surrogate not initialised") : AFQ_ERR_ARG;
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
+  ensure_theta(c);
   afq::launch_features(s, nullptr, s.W, c->cvX, 0, c->stream);
   afq::launch_predict(s, c->mlp, c->cvX, s.W, c->cvS, c->stream);
   launch_check("predict");
@@ -1524,6 +1543,7 @@ int afq_surrogate_features(afq_ctx* c, float* X) {
   AFQ_GUARD_BEGIN
   DevState& s = c->s;
   const int F = s.G + 2 * s.N;
+  ensure_theta(c);
   float* d = dalloc<float>((int64_t)s.W * F);
   afq::launch_features(s, nullptr, s.W, d, 0, c->stream);
   launch_check("features");

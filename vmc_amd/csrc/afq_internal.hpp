@@ -89,6 +89,10 @@ struct DevState {
   // force bias; cleared by whatever changes theta or the weights), so the exchange of
   // the same walkers need not split again
   mutable bool tb_live = false;
+  // theta is not carried through an eager comb (nothing reads it before the next
+  // step's Green's function recomputes it); entry points that read it first call
+  // ensure_theta, and the next propagation recomputes it for every walker
+  bool theta_stale = false;
   int prec_prop = AFQ_PREC_FP64;   // Taylor products V t
   // split-BF16 Taylor operands (allocated when prec_prop is BF16X3); orbitals come in
   // chunks of 16 (Nq = N padded to 16, pads stay zero):

@@ -3037,7 +3037,8 @@ void launch_comb(DevState& s, double zeta, hipStream_t st) {
     }
   };
   gather(s.psi, s.psi2, 2 * s.ms);
-  gather(s.theta, s.theta2, 2 * s.Ktot);
+  if (s.graph_mode) gather(s.theta, s.theta2, 2 * s.Ktot);
+  else s.theta_stale = true;
   gather(s.fb, s.fb2, 2 * (int64_t)s.G);
   gather(s.logO, s.logO2, 2);
   gather(s.e1, s.e12, 2 * (int64_t)s.nslots);
