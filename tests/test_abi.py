@@ -58,3 +58,14 @@ def test_struct_layout_matches_header(tmp_path, cname, pyname):
     size, offs = c_layout(tmp_path, cname, fields)
     assert C.sizeof(py) == size
     assert [getattr(py, f).offset for f in fields] == offs
+
+
+def test_io_library_exports_its_header():
+    io_lib = os.path.join(ROOT, "vmc_amd", "libvmc_fcidump.so")
+    if not os.path.exists(io_lib):
+        pytest.skip("libvmc_fcidump.so not built (no HDF5 C library)")
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "vmc_afqmc_io.h")).read(), flags=re.S)
+    names = sorted(set(re.findall(r"^\s*\w+\s*\*?\s*(afq_\w+)\s*\(", text, flags=re.M)))
+    assert names == ["afq_fcidump_header", "afq_fcidump_read", "afq_fcidump_write"]
+    lib = C.CDLL(io_lib)
+    assert all(hasattr(lib, n) for n in names)

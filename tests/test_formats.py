@@ -49,3 +49,66 @@ def test_samples_file_format(tmp_path):
     write_samples(str(p), [80.0, 79.51234567], [-75.9839842460, -76.1])
     assert p.read_text() == ("8.0000000e+01      -7.5983984246e+01 \n"
                              "7.9512346e+01      -7.6100000000e+01 \n")
+
+
+# ---------------------------------------------------------------- FCIDUMP_chol (HDF5)
+import os  # noqa: E402
+import shutil  # noqa: E402
+
+import pytest  # noqa: E402
+
+from vmc_amd.formats import read_fcidump_chol, write_fcidump_chol  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+REF_TESTS = "/root/reference/tests/DQMC"   # read only when present (this container), never on a GPU run
+io_built = pytest.mark.skipif(
+    not os.path.exists(os.path.join(os.path.dirname(GOLD), "..", "vmc_amd", "libvmc_fcidump.so")),
+    reason="libvmc_fcidump.so not built (no HDF5 C library)")
+
+
+@io_built
+def test_fcidump_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
+    N, G = 7, 5
+    h1 = rng.standard_normal((N, N))
+    h1m = rng.standard_normal((N, N))
+    L = rng.standard_normal((G, N, N))
+    L = L + L.transpose(0, 2, 1)
+    p = str(tmp_path / "FCIDUMP_chol")
+    write_fcidump_chol(p, h1, h1m, L, nelec=9, ecore=3.25, ms=1)
+    d = read_fcidump_chol(p)
+    assert (d.nelec, d.norb, d.ms, d.nchol, d.nalpha, d.nbeta) == (9, N, 1, G, 5, 4)
+    assert np.array_equal(d.h1, h1) and np.array_equal(d.h1_mod, h1m) and np.array_equal(d.chol, L)
+    assert d.ecore == 3.25
+    if shutil.which("h5dump") or os.path.exists("/opt/conda/bin/h5dump"):
+        exe = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+        out = subprocess.run([exe, "-H", p], capture_output=True, text=True, check=True).stdout
+        # the reference writer's types and shapes (h5py from numpy int64 / float64 arrays)
+        assert 'DATASET "header"' in out and "H5T_STD_I64LE" in out and "( 4 ) / ( 4 )" in out
+        assert 'DATASET "chol"' in out and "( %d ) / ( %d )" % (G * N * N, G * N * N) in out
+        assert "SCALAR" in out
+
+
+@io_built
+def test_fcidump_missing_file(tmp_path):
+    with pytest.raises(RuntimeError):
+        read_fcidump_chol(str(tmp_path / "absent"))
+
+
+@io_built
+@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_uhf", "rhf_uhf", "uhf_rhf", "uhf_uhf_ui"])
+def test_fcidump_reads_reference_files(name):
+    """The reference's own FCIDUMP_chol files, read directly, equal the committed fixtures
+    (which h5dump extracted from the same files)."""
+    path = os.path.join(REF_TESTS, name, "FCIDUMP_chol")
+    if not os.path.exists(path):
+        pytest.skip("reference test files not present")
+    z = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    if "hcore_mod" not in z.files:
+        pytest.skip("fixture holds no restricted integrals")
+    d = read_fcidump_chol(path)
+    assert [d.nelec, d.norb, d.ms, d.nchol] == [int(v) for v in z["header"]]
+    assert np.array_equal(d.h1.ravel(), np.asarray(z["hcore"]).ravel())
+    assert np.array_equal(d.h1_mod.ravel(), np.asarray(z["hcore_mod"]).ravel())
+    assert np.array_equal(d.chol.ravel(), np.asarray(z["chol"]).ravel())
+    assert d.ecore == float(np.asarray(z["energy_core"]).ravel()[0])

@@ -22,6 +22,11 @@ ARCH = "gfx950"
 SOURCES = ["afq_kernels.hip", "afq_mlp.hip", "afq_ctx.cpp", "afq_driver.cpp"]
 HEADERS = ["afq_device.hpp", "afq_internal.hpp"]
 CPP_TESTS = [("tests/cpp/api_parity.cpp", "tests/cpp/api_parity")]  # host programs over the C++ mirror
+# FCIDUMP_chol (HDF5) reader/writer, a host library of its own: built where the HDF5 C
+# library is installed (HDF5_DIR, default /opt/conda), skipped otherwise
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+IO_SRC = os.path.join(CSRC, "afq_fcidump.cpp")
+IO_LIB = os.path.join(HERE, "libvmc_fcidump.so")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(ROOT, "include")]
@@ -61,6 +66,7 @@ def build(force=False, verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(LIB + ".tmp", LIB)
+    build_io(verbose)
     for src, exe in CPP_TESTS:
         cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, src),
                "-o", os.path.join(ROOT, exe), "-L" + HERE, "-lvmc_afqmc", "-Wl,-rpath,$ORIGIN/../../vmc_amd"]
@@ -68,6 +74,22 @@ def build(force=False, verbose=True):
         if r.returncode != 0:
             raise RuntimeError("g++ failed for %s:\n%s" % (src, r.stderr))
     return LIB
+
+
+def build_io(verbose=True):
+    """libvmc_fcidump.so (g++, host only); returns its path, or None without HDF5."""
+    inc, lib = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print("[build] no hdf5.h under %s: FCIDUMP_chol reader not built" % HDF5_DIR, file=sys.stderr)
+        return None
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-fPIC", "-shared", "-I" + inc, "-I" + os.path.join(ROOT, "include"),
+           IO_SRC, "-o", IO_LIB + ".tmp", "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed for afq_fcidump.cpp:\n" + r.stderr)
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

@@ -5,6 +5,8 @@ A case mirrors one reference test directory: the FCIDUMP_chol fields
 (utils/input.cpp:376-383, RHF N x N, UHF N x 2N with alpha columns first),
 the afqmc.json sampling keys, and samples.ref (DQMC/MixedEstimator.cpp:678-683).
 """
+import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -100,3 +102,69 @@ def write_one_rdm(scratch_dir, rdm, cumulative_weight, rank=0):
                 f.write("".join("%g  " % v for v in row) + "\n")
         paths.append(path)
     return paths
+
+
+# ---------------------------------------------------------------- FCIDUMP_chol (HDF5)
+_IO_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libvmc_fcidump.so")
+_io = None
+
+
+def _io_lib():
+    global _io
+    if _io is None:
+        if not os.path.exists(_IO_LIB):
+            raise RuntimeError("%s is not built (needs the HDF5 C library; run python -m vmc_amd.build)" % _IO_LIB)
+        L = C.CDLL(_IO_LIB)
+        DP, LP = C.POINTER(C.c_double), C.POINTER(C.c_int64)
+        L.afq_fcidump_header.argtypes = [C.c_char_p, LP]
+        L.afq_fcidump_read.argtypes = [C.c_char_p, DP, DP, DP, DP]
+        L.afq_fcidump_write.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, DP, DP, DP,
+                                        C.c_double]
+        _io = L
+    return _io
+
+
+@dataclass
+class Integrals:
+    nelec: int
+    norb: int
+    ms: int
+    nchol: int
+    nalpha: int
+    nbeta: int
+    h1: np.ndarray       # norb x norb
+    h1_mod: np.ndarray   # norb x norb
+    chol: np.ndarray     # nchol x norb x norb
+    ecore: float
+
+
+def read_fcidump_chol(path):
+    """readDQMCIntegralsRG (utils/integral.cpp:589-661): the reference's Cholesky
+    integral file; nalpha = (nelec + ms) / 2 (:614-616)."""
+    L = _io_lib()
+    hd = np.zeros(4, np.int64)
+    rc = L.afq_fcidump_header(path.encode(), hd.ctypes.data_as(C.POINTER(C.c_int64)))
+    if rc:
+        raise RuntimeError("afq_fcidump_header(%s): error %d" % (path, rc))
+    nelec, N, ms, G = (int(v) for v in hd)
+    h1, h1m, chol, ec = np.zeros((N, N)), np.zeros((N, N)), np.zeros((G, N, N)), np.zeros(1)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    rc = L.afq_fcidump_read(path.encode(), dp(h1), dp(h1m), dp(chol), dp(ec))
+    if rc:
+        raise RuntimeError("afq_fcidump_read(%s): error %d" % (path, rc))
+    na = (nelec + ms) // 2
+    return Integrals(nelec, N, ms, G, na, nelec - na, h1, h1m, chol, float(ec[0]))
+
+
+def write_fcidump_chol(path, h1, h1_mod, chol, nelec, ecore, ms=0):
+    """write_dqmc (scripts/QMCUtils.py:1067-1075): chol is nchol x norb x norb (or
+    nchol x norb^2)."""
+    h1 = np.ascontiguousarray(h1, dtype=np.float64)
+    N = h1.shape[0]
+    h1m = np.ascontiguousarray(h1_mod, dtype=np.float64).reshape(N, N)
+    chol = np.ascontiguousarray(chol, dtype=np.float64).reshape(-1, N * N)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    rc = _io_lib().afq_fcidump_write(path.encode(), int(nelec), N, int(ms), chol.shape[0], dp(h1), dp(h1m),
+                                     dp(chol), float(ecore))
+    if rc:
+        raise RuntimeError("afq_fcidump_write(%s): error %d" % (path, rc))
