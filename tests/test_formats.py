@@ -2,6 +2,7 @@
 for the same numbers (writeOneRDM, DQMC/MixedEstimator.cpp:52-112: default
 stream format, entries followed by two spaces), and samples.dat matches
 boost::format("%.7e      %.10e \\n") (DQMC/MixedEstimator.cpp:681), which is C's printf."""
+import json
 import subprocess
 
 import numpy as np
@@ -112,3 +113,28 @@ def test_fcidump_reads_reference_files(name):
     assert np.array_equal(d.h1_mod.ravel(), np.asarray(z["hcore_mod"]).ravel())
     assert np.array_equal(d.chol.ravel(), np.asarray(z["chol"]).ravel())
     assert d.ecore == float(np.asarray(z["energy_core"]).ravel()[0])
+
+
+# ---------------------------------------------------------------- afqmc.json
+from vmc_amd.dqmc import read_afqmc_json  # noqa: E402
+
+
+def test_afqmc_json_keys_and_defaults(tmp_path):
+    """utils/input.cpp:61-276: the keys the phaseless walk reads, case-insensitive
+    (boost iptree), with the reference's defaults where absent."""
+    p = tmp_path / "afqmc.json"
+    p.write_text(json.dumps({"System": {"integrals": "FCIDUMP_chol"},
+                             "wavefunction": {"left": "UHF", "right": "rhf"},
+                             "sampling": {"seed": 4321, "Phaseless": True, "dt": 0.005, "nsteps": 50,
+                                          "nwalk": 20, "stochasticIter": 50, "choleskyThreshold": 0.001,
+                                          "orthoSteps": 20},
+                             "print": {"writeOneRDM": True}}))
+    s = read_afqmc_json(str(p))
+    assert (s.integrals, s.int_type, s.left, s.right) == ("FCIDUMP_chol", "r", "uhf", "rhf")
+    assert (s.seed, s.phaseless, s.dt, s.nsteps, s.nwalk) == (4321, True, 0.005, 50, 20)
+    assert (s.stochastic_iter, s.cholesky_threshold, s.ortho_steps) == (50, 0.001, 20)
+    assert (s.burn_iter, s.ene0_guess, s.weight_cap, s.write_one_rdm, s.scratch_dir) == (0, 1e10, -1.0, True, "./")
+    p.write_text(json.dumps({"sampling": {"seed": 1}}))
+    d = read_afqmc_json(str(p))
+    assert (d.integrals, d.dt, d.nsteps, d.nwalk, d.stochastic_iter, d.ortho_steps, d.cholesky_threshold,
+            d.phaseless) == ("FCIDUMP", 0.1, 10, 100, 10000, 50, 0.005, False)

@@ -1,0 +1,70 @@
+"""The reference's input directory run as the DQMC binary would run it
+(vmc_amd.dqmc.run_directory): an afqmc.json, an HDF5 FCIDUMP_chol and the
+rhf.txt / uhf.txt trial text, all in the reference's formats, rebuilt here from
+the committed fixtures (the same numbers as the reference's tests/DQMC files).
+With 4 virtual ranks (tests/runDQMC.sh:6 runs mpirun -np 4) the GPU reproduces
+the reference's samples.ref within its own tolerance, 1e-7 (tests/runDQMC.sh:9),
+on the first 10 blocks, and writes samples.dat in the reference's format.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from vmc_amd.formats import load_case, write_fcidump_chol
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def make_directory(tmp_path, name, write_rdm=False):
+    z = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+    c = load_case(os.path.join(GOLD, name + ".npz"))
+    h = c.ham
+    nelec, N, ms, G = (int(v) for v in z["header"])
+    try:
+        write_fcidump_chol(str(tmp_path / "FCIDUMP_chol"), h.h1, h.h1_mod, h.chol, nelec=nelec,
+                           ecore=h.ecore, ms=ms)
+    except (OSError, RuntimeError) as e:
+        pytest.skip("FCIDUMP_chol writer unavailable: %s" % e)
+    for kind in ("rhf", "uhf"):
+        if kind in z.files:
+            np.savetxt(str(tmp_path / (kind + ".txt")), np.asarray(z[kind]), fmt="%.17g")
+    cfg = {"system": {"integrals": "FCIDUMP_chol"},
+           "wavefunction": {"left": c.left, "right": c.right},
+           "sampling": {"seed": c.seed, "phaseless": True, "dt": c.dt, "nsteps": c.nsteps, "nwalk": c.nwalk,
+                        "stochasticIter": c.nsweeps, "choleskyThreshold": c.cholesky_threshold,
+                        "orthoSteps": c.ortho_steps},
+           "print": {"writeOneRDM": write_rdm}}
+    (tmp_path / "afqmc.json").write_text(json.dumps(cfg))
+    return c
+
+
+@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_uhf"])
+def test_run_directory_reproduces_samples_ref(tmp_path, name):
+    from vmc_amd.dqmc import run_directory
+    c = make_directory(tmp_path, name)
+    nb = 10
+    w, e, res, rdm = run_directory(str(tmp_path), nranks=c.nranks, max_steps=nb * c.nsteps - 1)
+    assert rdm is None
+    assert np.all(np.abs(e[:nb] - c.samples[:nb, 1]) < 1e-7), np.abs(e[:nb] - c.samples[:nb, 1])
+    assert np.all(np.abs(w[:nb] - c.samples[:nb, 0]) < 1e-6 * c.samples[:nb, 0])
+    lines = (tmp_path / "samples.dat").read_text().splitlines()
+    assert len(lines) == c.nsweeps
+    for i in range(nb):
+        wt, en = (float(x) for x in lines[i].split())
+        assert abs(en - c.samples[i, 1]) < 1e-7 and abs(wt - c.samples[i, 0]) < 1e-6 * c.samples[i, 0]
+
+
+def test_run_directory_writes_one_rdm(tmp_path):
+    from vmc_amd.dqmc import run_directory
+    c = make_directory(tmp_path, "rhf_rhf", write_rdm=True)
+    w, e, res, rdm = run_directory(str(tmp_path), nranks=c.nranks)
+    assert res.one_rdm_weight > 0
+    text = (tmp_path / "rdm_0.dat").read_text().splitlines()
+    assert abs(float(text[0]) - res.one_rdm_weight) <= 1e-5 * res.one_rdm_weight
+    m = np.array([[float(x) for x in ln.split()] for ln in text[1:]])
+    assert m.shape == (c.ham.norb, c.ham.norb)
+    assert np.allclose(m, rdm, rtol=1e-5, atol=1e-6)   # 6 significant digits in the file
+    assert abs(np.trace(rdm) - (c.ham.nalpha + c.ham.nbeta)) < 1e-9
