@@ -370,6 +370,9 @@ typedef struct afq_run_params {
                            * (DQMC/MixedEstimator.cpp:492-510): N x N spin-summed for an RHF
                            * trial (RHF::oneRDM), 2 x N x N up then down for a UHF trial
                            * (UHF::oneRDM); its cumulative weight is one_rdm_weight          */
+  const double* rdm_background; /* non-null: 2 x N x N (up, down) background density for the
+                           * mean-field shifts in place of the reference determinant's, as
+                           * prepPropR reads spinRDM.0.0.txt (DQMC/DQMCWalker.cpp:42-58)      */
 } afq_run_params;
 
 typedef struct afq_run_result {

@@ -138,3 +138,34 @@ def test_afqmc_json_keys_and_defaults(tmp_path):
     d = read_afqmc_json(str(p))
     assert (d.integrals, d.dt, d.nsteps, d.nwalk, d.stochastic_iter, d.ortho_steps, d.cholesky_threshold,
             d.phaseless) == ("FCIDUMP", 0.1, 10, 100, 10000, 50, 0.005, False)
+
+
+# ---------------------------------------------------------------- spinRDM.0.0.txt
+from vmc_amd.formats import read_spin_rdm  # noqa: E402
+
+
+def test_spin_rdm_of_a_determinant(tmp_path):
+    """For one determinant with spin-orbital density D (idempotent, trace n) the
+    2-RDM is Gamma_abcd = D_ac D_bd - D_ad D_bc, and its contraction over the second
+    index divided by n - 1 is D again (closed form); listing only a < b, c < d, as
+    the file does, must give the same.  Mixed spin blocks of D vanish for an
+    unrestricted determinant, and the reader returns the even/odd blocks."""
+    rng = np.random.default_rng(6)
+    N, na, nb = 5, 3, 2
+    Cu = np.linalg.qr(rng.standard_normal((N, na)))[0]
+    Cd = np.linalg.qr(rng.standard_normal((N, nb)))[0]
+    Du, Dd = Cu @ Cu.T, Cd @ Cd.T
+    D = np.zeros((2 * N, 2 * N))
+    D[0::2, 0::2], D[1::2, 1::2] = Du, Dd
+    lines = []
+    for a in range(2 * N):
+        for b in range(a + 1, 2 * N):
+            for c in range(2 * N):
+                for d in range(c + 1, 2 * N):
+                    g = D[a, c] * D[b, d] - D[a, d] * D[b, c]
+                    if abs(g) > 0:
+                        lines.append("%d %d %d %d %.17g" % (a, b, c, d, g))
+    p = tmp_path / "spinRDM.0.0.txt"
+    p.write_text("\n".join(lines) + "\n")
+    gu, gd = read_spin_rdm(str(p), N, na + nb)
+    assert np.allclose(gu, Du, atol=1e-13) and np.allclose(gd, Dd, atol=1e-13)

@@ -68,3 +68,29 @@ def test_run_directory_writes_one_rdm(tmp_path):
     assert m.shape == (c.ham.norb, c.ham.norb)
     assert np.allclose(m, rdm, rtol=1e-5, atol=1e-6)   # 6 significant digits in the file
     assert abs(np.trace(rdm) - (c.ham.nalpha + c.ham.nbeta)) < 1e-9
+
+
+def test_run_directory_spin_rdm_background(tmp_path):
+    """With a spinRDM.0.0.txt holding the 2-RDM of the reference determinant itself,
+    prepPropR's file branch (DQMC/DQMCWalker.cpp:42-58) must give the background the
+    determinant gives, so the run still reproduces samples.ref."""
+    from vmc_amd.dqmc import run_directory
+    c = make_directory(tmp_path, "rhf_rhf")
+    N = c.ham.norb
+    D = np.zeros((2 * N, 2 * N))
+    for s, r in enumerate((c.ref_up, c.ref_dn)):
+        D[s::2, s::2] = r @ np.linalg.solve(r.T @ r, r.T)
+    idx = np.argwhere(np.abs(D) > 0)
+    lines = []
+    for a in range(2 * N):
+        for b in range(a + 1, 2 * N):
+            for cc in range(2 * N):
+                for d in range(cc + 1, 2 * N):
+                    g = D[a, cc] * D[b, d] - D[a, d] * D[b, cc]
+                    if g != 0.0:
+                        lines.append("%d %d %d %d %.17g" % (a, b, cc, d, g))
+    assert len(idx) > 0
+    (tmp_path / "spinRDM.0.0.txt").write_text("\n".join(lines) + "\n")
+    nb = 10
+    w, e, res, _ = run_directory(str(tmp_path), nranks=c.nranks, max_steps=nb * c.nsteps - 1)
+    assert np.all(np.abs(e[:nb] - c.samples[:nb, 1]) < 1e-7), np.abs(e[:nb] - c.samples[:nb, 1])

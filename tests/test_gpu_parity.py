@@ -305,3 +305,34 @@ def test_largest_shape_parity():
         else:  # no local energy for a dead walker (nor re-orthonormalisation, so its
             # overlap keeps the scale of its last propagation)
             assert ho[w][0] == 0 and e[w] == 0
+
+
+@pytest.mark.parametrize("name", ["uhf_uhf", "rhf_rhf", "uhf_33_31"])
+def test_background_rdm_step(name):
+    """prepPropR's spinRDM.0.0.txt branch (DQMC/DQMCWalker.cpp:42-58): a background
+    density that is not the reference determinant's (symmetrised, shifted Gref) gives
+    the same step on the GPU as in the oracle."""
+    obj, wrhf = SYSTEMS[name]
+    obj = obj()
+    if hasattr(obj, "ham"):
+        h, wrhf, ru, rd = obj.ham, obj.walker_rhf, obj.ref_up, obj.ref_dn
+    else:
+        h, ru, rd = obj, obj.phi_up, obj.phi_dn
+    W = 6
+    G = [r @ np.linalg.solve(r.T @ r, r.T) for r in (ru, rd)]
+    G = [0.5 * (g + g.T) + 0.03 * np.eye(h.norb) for g in G]
+    ctx = Context(h, W, wrhf)
+    orc = O.Oracle(h, W, wrhf)
+    for side in (ctx, orc):
+        side.prep_prop(ru, rd, 0.005, rdm_up=G[0], rdm_dn=G[1])
+    rng = np.random.default_rng(13)
+    dets = perturbed_walkers(h, W, wrhf, rng, scale=0.1)
+    ctx.set_walkers(dets)
+    orc.set_walkers(dets)
+    eshift = float(np.mean([(lambda o: (o[0] / o[1]).real)(orc.ham_and_overlap(w)) for w in range(W)]))
+    x = rng.standard_normal((W, h.nchol))
+    fg, _ = ctx.propagate(x, eshift)
+    fo, _ = orc.propagate(x, eshift, 100.0)
+    assert rel(fg, fo) < TOL, rel(fg, fo)
+    check_state(h, ctx, orc, what="background rdm")
+    ctx.close()

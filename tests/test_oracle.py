@@ -368,3 +368,26 @@ def test_one_rdm_running_average_over_measurements():
     ref = np.tensordot(np.array(weights), np.array(samples), axes=1) / np.sum(weights)
     assert abs(state[1] - np.sum(weights)) < 1e-13 * np.sum(weights)
     assert np.allclose(state[0], ref, rtol=0, atol=1e-13)
+
+
+def test_prep_prop_background_rdm_path():
+    """prepPropR's spinRDM.0.0.txt branch (DQMC/DQMCWalker.cpp:42-58) takes the
+    background density from the file instead of Gref = ref (ref^T ref)^-1 ref^T; fed
+    exactly that Gref it must give the same shifts, propagator and constant, and fed
+    another density, shifts mf_g = i sum_s tr(G_s L_g) (closed form)."""
+    h = small_system("uhf")
+    rng = np.random.default_rng(12)
+    ru = h.phi_up + 0.1 * rng.standard_normal(h.phi_up.shape)
+    rd = h.phi_dn + 0.1 * rng.standard_normal(h.phi_dn.shape)
+    gref = [r @ np.linalg.solve(r.T @ r, r.T) for r in (ru, rd)]
+    a, b = O.Oracle(h, 1, False), O.Oracle(h, 1, False)
+    a.prep_prop(ru, rd, 0.01)
+    b.prep_prop(ru, rd, 0.01, rdm_up=gref[0], rdm_dn=gref[1])
+    for x, y in zip(a.prop_data(), b.prop_data()):
+        assert np.allclose(x, y, rtol=1e-12, atol=1e-13)
+    G = [0.5 * (g + g.T) + 0.05 * np.eye(h.norb) for g in gref]
+    c = O.Oracle(h, 1, False)
+    c.prep_prop(ru, rd, 0.01, rdm_up=G[0], rdm_dn=G[1])
+    mf, _, _ = c.prop_data()
+    ref_mf = 1j * np.einsum("gij,ij->g", h.chol, G[0] + G[1])
+    assert np.allclose(mf, ref_mf, rtol=1e-12, atol=1e-13)

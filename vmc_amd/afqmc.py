@@ -440,6 +440,7 @@ class RunParams(C.Structure):
         ("surrogate_epochs", C.c_int32), ("subset_inv", C.c_int32), ("fallback_rms", C.c_double),
         ("risk_frac", C.c_double), ("shadow_exact", C.c_int32), ("surrogate_kind", C.c_int32),
         ("surrogate_prec", C.c_int32), ("prec_prop", C.c_int32), ("one_rdm_out", DP),
+        ("rdm_background", DP),
     ]
 
 
@@ -463,10 +464,11 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   energy_mode=AFQ_ENERGY_EXACT, device=0, surrogate_train_steps=0, surrogate_hidden=256,
                   surrogate_epochs=20, subset_inv=16, fallback_rms=0.0, risk_frac=0.05, shadow_exact=False,
                   surrogate_kind=AFQ_SURR_MLP, surrogate_prec=AFQ_PREC_BF16, prec_prop=AFQ_PREC_FP64,
-                  one_rdm=None):
+                  one_rdm=None, background_rdm=None):
     """calcMixedEstimatorLongProp on the GPU; returns (weights, energies, RunResult).
     one_rdm: None, or a float64 array that receives the mixed-estimator 1-RDM
-    (N x N for an RHF trial, 2 x N x N for a UHF trial)."""
+    (N x N for an RHF trial, 2 x N x N for a UHF trial).  background_rdm: None, or
+    the 2 x N x N (up, down) density of spinRDM.0.0.txt for the mean-field shifts."""
     L = lib()
     if not hasattr(L, "_run_bound"):
         L.afq_run_long_prop.argtypes = [C.POINTER(AfqHam), C.POINTER(AfqTrial), DP, DP, DP, DP,
@@ -477,7 +479,8 @@ def run_long_prop(ham, ref_up, ref_dn, init_up, init_dn, *, seed, dt, nsteps, ns
                   int(ref_compat), cholesky_threshold, ene0_guess, weight_cap, max_steps, rng_mode,
                   warmup_steps, prec_fb, prec_vhs, prec_exchange, energy_mode, device,
                   surrogate_train_steps, surrogate_hidden, surrogate_epochs, subset_inv, fallback_rms,
-                  risk_frac, int(shadow_exact), surrogate_kind, surrogate_prec, prec_prop, dptr(one_rdm))
+                  risk_frac, int(shadow_exact), surrogate_kind, surrogate_prec, prec_prop, dptr(one_rdm),
+                  dptr(None if background_rdm is None else np.ascontiguousarray(background_rdm, dtype=np.float64)))
     w = np.zeros(nsweeps)
     e = np.zeros(nsweeps)
     res = RunResult()

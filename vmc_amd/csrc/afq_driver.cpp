@@ -96,7 +96,8 @@ extern "C" int afq_run_long_prop(const afq_ham* ham, const afq_trial* left, cons
     }
   }
   if ((rc = afq_set_nchol_ene(c, nchol))) return rc;
-  if ((rc = afq_prep_prop(c, ref_up, ref_dn, nullptr, nullptr, dt, ene0))) return rc;
+  const double* bg = p->rdm_background;
+  if ((rc = afq_prep_prop(c, ref_up, ref_dn, bg, bg ? bg + (size_t)N * N : nullptr, dt, ene0))) return rc;
   std::vector<double> dets((size_t)W * 2 * ms);
   for (int w = 0; w < W; ++w) std::memcpy(&dets[(size_t)w * 2 * ms], init.data(), 2 * ms * sizeof(double));
   if ((rc = afq_set_walkers(c, dets.data(), nullptr, nullptr))) return rc;

@@ -4,7 +4,8 @@ Reads what executables/DQMC.cpp reads -- the JSON input (utils/input.cpp:61-276,
 the keys the phaseless walk uses, with the reference's defaults; keys are
 case-insensitive as in its boost iptree), the FCIDUMP_chol integrals
 (utils/integral.cpp:589-661) and the rhf.txt / uhf.txt trial text
-(utils/input.cpp:376-383) -- picks the walker type and trials as
+(utils/input.cpp:376-383), and spinRDM.0.0.txt where present (the background
+density, DQMC/DQMCWalker.cpp:42-58) -- picks the walker type and trials as
 executables/DQMC.cpp:74-185 does, runs calcMixedEstimatorLongProp on the GPU
 (afq_run_long_prop) and writes samples.dat (DQMC/MixedEstimator.cpp:678-683)
 and, with print.writeOneRDM, the 1-RDM files (:689-697).
@@ -20,7 +21,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from .abi import HamArrays
-from .formats import read_fcidump_chol, read_text_matrix, trial_orbitals, walker_is_rhf, write_one_rdm, write_samples
+from .formats import (read_fcidump_chol, read_spin_rdm, read_text_matrix, trial_orbitals, walker_is_rhf,
+                      write_one_rdm, write_samples)
 
 
 @dataclass
@@ -110,6 +112,12 @@ def run_directory(directory, input_file="afqmc.json", *, nranks=1, ref_compat=Tr
     # reference dets: rhf.txt for an RHF walker, uhf.txt otherwise (DQMC/MixedEstimator.cpp:338-351)
     ru, rd = trial_orbitals("rhf" if wrhf else "uhf", mats, N, na, nb)
     iu, idn = trial_orbitals(s.right, mats, N, na, nb)
+    # prepPropR takes its background density from spinRDM.0.0.txt when the file exists
+    # (DQMC/DQMCWalker.cpp:42-58)
+    bg = None
+    p = os.path.join(directory, "spinRDM.0.0.txt")
+    if os.path.exists(p):
+        bg = np.array(read_spin_rdm(p, N, na + nb))
     rdm = None
     if s.write_one_rdm:
         rdm = np.zeros((N, N) if s.left == "rhf" else (2, N, N))
@@ -118,7 +126,7 @@ def run_directory(directory, input_file="afqmc.json", *, nranks=1, ref_compat=Tr
                               ref_compat=ref_compat, cholesky_threshold=s.cholesky_threshold,
                               burn_iter=s.burn_iter, ene0_guess=s.ene0_guess,
                               weight_cap=s.weight_cap if s.weight_cap > 0 else 0.0, max_steps=max_steps,
-                              one_rdm=rdm, **run_kw)
+                              one_rdm=rdm, background_rdm=bg, **run_kw)
     if write:
         write_samples(os.path.join(directory, "samples.dat"), w, e)
         if rdm is not None:

@@ -78,6 +78,34 @@ def read_text_matrix(path, rows, cols):
     return np.loadtxt(path).reshape(rows, cols)
 
 
+def read_spin_rdm(path, norb, nelec):
+    """The one-body part of readSpinRDM (utils/input.cpp:1043-1090) as prepPropR
+    splits it (DQMC/DQMCWalker.cpp:42-58): each line "a b c d elem" of the
+    spin-orbital 2-RDM adds elem to the 1-RDM at (a,c) if b == d, (b,d) if a == c
+    and subtracts it at (a,d) if b == c, (b,c) if a == d; the sum is divided by
+    nelec - 1.  Spin orbital 2i is orbital i up, 2i+1 orbital i down.  Returns the
+    (up, down) N x N blocks."""
+    ns = 2 * norb
+    one = np.zeros((ns, ns))
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) < 5:
+                continue
+            a, b, c, d = (int(x) for x in w[:4])
+            e = float(w[4])
+            if b == d:
+                one[a, c] += e
+            if b == c:
+                one[a, d] -= e
+            if a == d:
+                one[b, c] -= e
+            if a == c:
+                one[b, d] += e
+    one /= nelec - 1
+    return one[0::2, 0::2].copy(), one[1::2, 1::2].copy()
+
+
 def write_samples(path, weights, energies):
     """samples.dat writer, DQMC/MixedEstimator.cpp:678-683 ("%.7e      %.10e ")."""
     with open(path, "w") as f:
