@@ -88,6 +88,15 @@ typedef struct afq_ham {
   const double* h1;     /* hcore, N x N                      */
   const double* h1_mod; /* hcore_mod = hcore - 1/2 sum_g L_g L_g, N x N */
   const double* chol;   /* G x N x N, row g = vec(L_g), each L_g symmetric */
+  /* Unrestricted integrals (intType "u": /hcore_up, /hcore_dn, /hcore_mod_up,
+   * /hcore_mod_dn, /chol_up, /chol_dn, utils/integral.cpp:664-741): with h1_dn,
+   * h1_mod_dn and chol_dn non-null, h1, h1_mod and chol are the up-spin ones and
+   * the walk is propagatePhaselessU (DQMC/DQMCWalker.cpp:106-187, 506-554); this
+   * needs a UHF trial, UHF walkers and FP64 VHS and propagation.  All three null:
+   * restricted integrals. */
+  const double* h1_dn;
+  const double* h1_mod_dn;
+  const double* chol_dn;
 } afq_ham;
 
 /* Left trial (RHF or UHF, DQMC/RHF.h, DQMC/UHF.h).  Orbitals are real. */

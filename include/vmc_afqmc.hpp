@@ -70,7 +70,7 @@ class Hamiltonian {
   void setNcholEne(int n) { ncholEne = n; }
 
   afq_ham c_struct() const {
-    afq_ham h;
+    afq_ham h{};  // restricted integrals: the down-spin pointers stay null
     h.norb = norbs; h.nalpha = nalpha; h.nbeta = nbeta; h.nchol = nchol; h.ecore = ecore;
     h.h1 = h1.data(); h.h1_mod = h1Mod.data(); h.chol = chol.data();
     return h;

@@ -266,8 +266,14 @@ struct orc_ctx {
   int N = 0, na = 0, nb = 0, G = 0, GE = 0;
   double ecore = 0.0;
   MD h1, h1mod;
-  std::vector<double> L;   // G x N x N (symmetric L_g)
+  std::vector<double> L;   // G x N x N (symmetric L_g); the up-spin vectors with unrestricted integrals
   std::vector<float> Lf;   // floattenCholesky: g*T + i(i+1)/2 + j, j <= i
+  // unrestricted integrals (intType "u", utils/integral.cpp:664-741): the down-spin
+  // one-body matrices, Cholesky vectors (and their float copy) and propagator
+  bool uints = false;
+  MD h1d, h1modd, expOBd;
+  std::vector<double> Ld;
+  std::vector<float> Lfd;
   bool trial_rhf = true, walker_rhf = true, ref_compat = false;
   MD phi[2], phiT[2];
   std::vector<double> R[2];  // rotCholMat[s]: R[s][g*(n*N) + k*n + i] = (Phi^T L_g)(i,k)
@@ -287,6 +293,16 @@ struct orc_ctx {
     if (j > i) std::swap(i, j);
     const size_t T = (size_t)N * (N + 1) / 2;
     return (double)Lf[(size_t)g * T + (size_t)i * (i + 1) / 2 + j];
+  }
+  // spin s's vectors: the shared ones with restricted integrals
+  double Lval_s(int s, int g, int i, int j) const {
+    return (uints && s == 1) ? Ld[(size_t)g * N * N + (size_t)i * N + j] : Lval(g, i, j);
+  }
+  double Lfval_s(int s, int g, int i, int j) const {
+    if (!(uints && s == 1)) return Lfval(g, i, j);
+    if (j > i) std::swap(i, j);
+    const size_t T = (size_t)N * (N + 1) / 2;
+    return (double)Lfd[(size_t)g * T + (size_t)i * (i + 1) / 2 + j];
   }
 };
 
@@ -360,12 +376,14 @@ void ham_and_overlap(const orc_ctx* c, const Walker& W, int GE, cd out[2],
     O *= (s.mult == 2) ? d * d : d;
     // sum_ij G(i,j) h1(i,j), G = theta Phi^T
     const MD& P = c->phi[s.phi];
+    // unrestricted integrals: spin s's h1 (DQMC/UHF.cpp: the intType "u" one-body term)
+    const MD& h1 = (c->uints && si == 1) ? c->h1d : c->h1;
     cd acc(0.0, 0.0);
     for (int i = 0; i < N; ++i)
       for (int j = 0; j < N; ++j) {
         cd g(0.0, 0.0);
         for (int k = 0; k < s.n; ++k) g += theta[si](i, k) * P(j, k);
-        acc += g * c->h1(i, j);
+        acc += g * h1(i, j);
       }
     e1 += (double)s.mult * acc;
   }
@@ -439,39 +457,56 @@ double propagate_walker(const orc_ctx* c, Walker& W, const double* x, double esh
     shift += (x[g] - fieldShift) * c->mf[g];
     fbTerm += (x[g] * fieldShift - fieldShift * fieldShift / 2.0);
   }
-  // VHS: V = i sqrt(dt) sum_g field_g L_g  (DQMC/DQMCWalker.cpp:406-421,459-462)
-  MC V(N, N);
-  for (int i = 0; i < N; ++i)
-    for (int j = 0; j <= i; ++j) {
-      double pr = 0.0, pi = 0.0;
-      for (int g = 0; g < G; ++g) {
-        const double l = c->ref_compat ? c->Lfval(g, i, j) : c->Lval(g, i, j);
-        pr += fields[g].real() * l;
-        pi += fields[g].imag() * l;
+  // VHS: V = i sqrt(dt) sum_g field_g L_g  (DQMC/DQMCWalker.cpp:406-421,459-462); with
+  // unrestricted integrals one per spin from that spin's vectors (:525-530)
+  auto vhs = [&](int spin) {
+    MC V(N, N);
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j <= i; ++j) {
+        double pr = 0.0, pi = 0.0;
+        for (int g = 0; g < G; ++g) {
+          const double l = c->ref_compat ? c->Lfval_s(spin, g, i, j) : c->Lval_s(spin, g, i, j);
+          pr += fields[g].real() * l;
+          pi += fields[g].imag() * l;
+        }
+        if (c->ref_compat) {
+          pr = (double)(float)pr;
+          pi = (double)(float)pi;
+        }
+        const cd v = cd(-pi, pr) * sq;
+        V(i, j) = v;
+        V(j, i) = v;
       }
-      if (c->ref_compat) {
-        pr = (double)(float)pr;
-        pi = (double)(float)pi;
-      }
-      const cd v = cd(-pi, pr) * sq;
-      V(i, j) = v;
-      V(j, i) = v;
-    }
+    return V;
+  };
+  const MC V = vhs(0);
+  if (c->uints) {
+    // propagatePhaselessU (DQMC/DQMCWalker.cpp:532-540): det[s] <- E_s exp(V_s) E_s det[s]
+    const MC Vd = vhs(1);
+    W.det[0] = real_times(c->expOB, W.det[0]);
+    apply_exp(V, W.det[0]);
+    W.det[0] = real_times(c->expOB, W.det[0]);
+    W.det[1] = real_times(c->expOBd, W.det[1]);
+    apply_exp(Vd, W.det[1]);
+    W.det[1] = real_times(c->expOBd, W.det[1]);
+  }
   if (vhs_out)
     for (int i = 0; i < N; ++i)
       for (int j = 0; j < N; ++j) {
         vhs_out[2 * ((size_t)i * N + j)] = V(i, j).real();
         vhs_out[2 * ((size_t)i * N + j) + 1] = V(i, j).imag();
       }
-  W.det[0] = real_times(c->expOB, W.det[0]);
-  apply_exp(V, W.det[0]);
-  W.det[0] = real_times(c->expOB, W.det[0]);
-  if (c->walker_rhf) {
-    W.det[1] = W.det[0];
-  } else {
-    W.det[1] = real_times(c->expOB, W.det[1]);
-    apply_exp(V, W.det[1]);
-    W.det[1] = real_times(c->expOB, W.det[1]);
+  if (!c->uints) {
+    W.det[0] = real_times(c->expOB, W.det[0]);
+    apply_exp(V, W.det[0]);
+    W.det[0] = real_times(c->expOB, W.det[0]);
+    if (c->walker_rhf) {
+      W.det[1] = W.det[0];
+    } else {
+      W.det[1] = real_times(c->expOB, W.det[1]);
+      apply_exp(V, W.det[1]);
+      W.det[1] = real_times(c->expOB, W.det[1]);
+    }
   }
   const cd oldOverlap = W.trialOverlap;
   const cd newOverlap = overlap(c, W);
@@ -555,6 +590,12 @@ orc_ctx* orc_create(const afq_ham* ham, const afq_trial* left, int walker_rhf, i
   c->h1 = read_rowmajor(ham->h1, N, N);
   c->h1mod = read_rowmajor(ham->h1_mod, N, N);
   c->L.assign(ham->chol, ham->chol + (size_t)c->G * N * N);
+  c->uints = ham->h1_dn && ham->h1_mod_dn && ham->chol_dn;
+  if (c->uints) {
+    c->h1d = read_rowmajor(ham->h1_dn, N, N);
+    c->h1modd = read_rowmajor(ham->h1_mod_dn, N, N);
+    c->Ld.assign(ham->chol_dn, ham->chol_dn + (size_t)c->G * N * N);
+  }
   // floattenCholesky (DQMC/Hamiltonian.cpp:179-220)
   const size_t T = (size_t)N * (N + 1) / 2;
   c->Lf.resize((size_t)c->G * T);
@@ -562,6 +603,14 @@ orc_ctx* orc_create(const afq_ham* ham, const afq_trial* left, int walker_rhf, i
     size_t cnt = 0;
     for (int i = 0; i < N; ++i)
       for (int j = 0; j <= i; ++j) c->Lf[(size_t)g * T + cnt++] = (float)c->Lval(g, i, j);
+  }
+  if (c->uints) {  // the down-spin half of floatChol (DQMC/Hamiltonian.cpp:193-196)
+    c->Lfd.resize((size_t)c->G * T);
+    for (int g = 0; g < c->G; ++g) {
+      size_t cnt = 0;
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j <= i; ++j) c->Lfd[(size_t)g * T + cnt++] = (float)c->Lval_s(1, g, i, j);
+    }
   }
   c->trial_rhf = left->rhf != 0;
   c->walker_rhf = walker_rhf != 0;
@@ -571,13 +620,14 @@ orc_ctx* orc_create(const afq_ham* ham, const afq_trial* left, int walker_rhf, i
   for (int s = 0; s < 2; ++s) {
     c->phiT[s] = transpose(c->phi[s]);
     // rotateCholesky (DQMC/Hamiltonian.cpp:52-133): R_g = Phi^T L_g, col-major n x N
+    // (unrestricted integrals: spin s's vectors, :99-107)
     const int n = c->phi[s].c;
     c->R[s].assign((size_t)c->G * n * N, 0.0);
     for (int g = 0; g < c->G; ++g)
       for (int m = 0; m < N; ++m) {
         const double* prow = &c->phiT[s].a[0];  // phiT(i, m) at i + m*n
         for (int k = 0; k < N; ++k) {
-          const double l = c->Lval(g, m, k);
+          const double l = c->Lval_s(s, g, m, k);
           if (l == 0.0) continue;
           double* r = &c->R[s][(size_t)g * n * N + (size_t)k * n];
           for (int i = 0; i < n; ++i) r[i] += prow[i + (size_t)m * n] * l;
@@ -634,27 +684,30 @@ void orc_prep_prop(orc_ctx* c, const double* ref_up, const double* ref_dn,
         }
     }
   }
-  MD h1p = c->h1mod;
+  // restricted: one h1' from h1Mod; unrestricted (prepPropU, DQMC/DQMCWalker.cpp:141-180):
+  // one per spin from that spin's h1Mod and vectors, the shifts summing both spins
+  MD h1p[2] = {c->h1mod, c->uints ? c->h1modd : c->h1mod};
+  auto lv = [&](int spin, int g, int i, int j) {
+    return c->ref_compat ? c->Lfval_s(spin, g, i, j) : c->Lval_s(spin, g, i, j);
+  };
   cd constant = ene0 - c->ecore;
   c->mf.assign(c->G, cd(0.0));
   for (int g = 0; g < c->G; ++g) {
     double a = 0.0;
     for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) {
-        const double l = c->ref_compat ? c->Lfval(g, i, j) : c->Lval(g, i, j);
-        a += (green[0](i, j) + green[1](i, j)) * l;
-      }
+      for (int j = 0; j < N; ++j)
+        a += c->uints ? green[0](i, j) * lv(0, g, i, j) + green[1](i, j) * lv(1, g, i, j)
+                      : (green[0](i, j) + green[1](i, j)) * lv(0, g, i, j);
     const cd mfs = cd(0.0, a);  // sum G o (i L)
     constant -= mfs * mfs / 2.0;
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) {
-        const double l = c->ref_compat ? c->Lfval(g, i, j) : c->Lval(g, i, j);
-        h1p(i, j) -= (mfs * cd(0.0, l)).real();  // mf * i L is real
-      }
+    for (int spin = 0; spin < (c->uints ? 2 : 1); ++spin)
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) h1p[spin](i, j) -= (mfs * cd(0.0, lv(spin, g, i, j))).real();  // mf * i L is real
     c->mf[g] = mfs;
   }
   c->propConstant = constant - ene0;
-  c->expOB = expm_sym(h1p, -dt / 2.0);
+  c->expOB = expm_sym(h1p[0], -dt / 2.0);
+  if (c->uints) c->expOBd = expm_sym(h1p[1], -dt / 2.0);
   c->prepped = true;
 }
 

@@ -391,3 +391,55 @@ def test_prep_prop_background_rdm_path():
     mf, _, _ = c.prop_data()
     ref_mf = 1j * np.einsum("gij,ij->g", h.chol, G[0] + G[1])
     assert np.allclose(mf, ref_mf, rtol=1e-12, atol=1e-13)
+
+
+# ---------------------------------------------------------------- unrestricted integrals
+
+@pytest.mark.parametrize("ref_compat", [True, False])
+def test_samples_ref_unrestricted_integrals(ref_compat):
+    """propagatePhaselessU (DQMC/DQMCWalker.cpp:106-187, 506-554) with the reference's
+    tests/DQMC/uhf_uhf_ui inputs (intType "u": per-spin h1, h1Mod and Cholesky
+    vectors, two VHS per step), 4 virtual ranks: the whole samples.ref within the
+    reference's own tolerance, 1e-7 (tests/runDQMC.sh:9)."""
+    c = load_case(os.path.join(GOLD, "uhf_uhf_ui.npz"))
+    assert c.ham.unrestricted and not c.walker_rhf
+    rc, w, e, res = run_case(c, ref_compat)
+    assert rc == 0
+    assert w[0] == c.samples[0, 0] and abs(e[0] - c.samples[0, 1]) < 1e-8 * abs(c.samples[0, 1])
+    assert np.max(np.abs(e - c.samples[:, 1])) < 1e-7
+    assert np.max(np.abs(w - c.samples[:, 0]) / c.samples[:, 0]) < 1e-7
+
+
+def test_unrestricted_energy_bruteforce():
+    """E_L with spin-resolved integrals against the full ERI: E = sum_s tr(G_s h1_s) +
+    1/2 sum_{st} sum (ij|kl)_{st} G_s,ji G_t,lk - 1/2 sum_s sum (ij|kl)_{ss} G_s,jk G_s,li,
+    (ij|kl)_{st} = sum_g L_{s,g,ij} L_{t,g,kl} (UHF::hamAndOverlap, DQMC/UHF.cpp:74-106)."""
+    rng = np.random.default_rng(17)
+    base = small_system("uhf", na=3, nb=2)
+    N, G = base.norb, base.nchol
+    Ld = rng.standard_normal((G, N, N)) * 0.3 / np.sqrt(N)
+    Ld = Ld + Ld.transpose(0, 2, 1)
+    h1d = base.h1 + 0.1 * np.diag(rng.standard_normal(N))
+    from vmc_amd.abi import HamArrays
+    h = HamArrays(N, 3, 2, 0.7, base.h1, base.h1_mod, base.chol, base.phi_up, base.phi_dn, trial_rhf=False,
+                  h1_dn=h1d, h1_mod_dn=h1d - 0.5 * np.einsum("gij,gjk->ik", Ld, Ld), chol_dn=Ld)
+    orc = O.Oracle(h, 2, False)
+    dets = random_walkers(h, 2, rng)
+    orc.set_walkers(dets)
+    L = [h.chol, Ld]
+    h1 = [h.h1, h1d]
+    for w in range(2):
+        up, dn = h.split_det(dets[w])
+        Gs, O_ = [], 1
+        for psi, phi in ((up, h.phi_up), (dn, h.phi_dn)):
+            S = phi.T @ psi
+            O_ *= np.linalg.det(S)
+            Gs.append(psi @ np.linalg.inv(S) @ phi.T)
+        E = h.ecore + sum(np.sum(Gs[s] * h1[s]) for s in range(2))
+        for s in range(2):
+            for t in range(2):
+                E += 0.5 * np.einsum("gij,gkl,ji,lk->", L[s], L[t], Gs[s], Gs[t])
+            E -= 0.5 * np.einsum("gij,gkl,jk,li->", L[s], L[s], Gs[s], Gs[s])
+        ho = orc.ham_and_overlap(w)
+        assert abs(ho[1] - O_) < 1e-12 * abs(O_)
+        assert abs(ho[0] / ho[1] - E) < 1e-11 * abs(E)

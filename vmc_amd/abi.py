@@ -31,6 +31,9 @@ class AfqHam(C.Structure):
         ("h1", DP),
         ("h1_mod", DP),
         ("chol", DP),
+        ("h1_dn", DP),
+        ("h1_mod_dn", DP),
+        ("chol_dn", DP),
     ]
 
 
@@ -69,13 +72,22 @@ class HamArrays:
     """Owns the numpy buffers behind one AfqHam / AfqTrial pair."""
 
     def __init__(self, norb, nalpha, nbeta, ecore, h1, h1_mod, chol,
-                 phi_up, phi_dn=None, trial_rhf=False):
+                 phi_up, phi_dn=None, trial_rhf=False, h1_dn=None, h1_mod_dn=None, chol_dn=None):
+        """h1_dn, h1_mod_dn, chol_dn: unrestricted integrals (h1, h1_mod, chol are then
+        the up-spin ones), or None."""
         self.norb, self.nalpha, self.nbeta = int(norb), int(nalpha), int(nbeta)
         self.ecore = float(ecore)
         self.h1 = np.ascontiguousarray(h1, dtype=np.float64).reshape(norb, norb)
         self.h1_mod = np.ascontiguousarray(h1_mod, dtype=np.float64).reshape(norb, norb)
         self.chol = np.ascontiguousarray(chol, dtype=np.float64).reshape(-1, norb, norb)
         self.nchol = self.chol.shape[0]
+        self.unrestricted = chol_dn is not None
+        if self.unrestricted:
+            self.h1_dn = np.ascontiguousarray(h1_dn, dtype=np.float64).reshape(norb, norb)
+            self.h1_mod_dn = np.ascontiguousarray(h1_mod_dn, dtype=np.float64).reshape(norb, norb)
+            self.chol_dn = np.ascontiguousarray(chol_dn, dtype=np.float64).reshape(self.nchol, norb, norb)
+        else:
+            self.h1_dn = self.h1_mod_dn = self.chol_dn = None
         self.trial_rhf = bool(trial_rhf)
         self.phi_up = np.ascontiguousarray(phi_up, dtype=np.float64).reshape(norb, nalpha)
         if phi_dn is None:
@@ -88,7 +100,8 @@ class HamArrays:
 
     def ham_struct(self):
         return AfqHam(self.norb, self.nalpha, self.nbeta, self.nchol, self.ecore,
-                      dptr(self.h1), dptr(self.h1_mod), dptr(self.chol))
+                      dptr(self.h1), dptr(self.h1_mod), dptr(self.chol),
+                      dptr(self.h1_dn), dptr(self.h1_mod_dn), dptr(self.chol_dn))
 
     def trial_struct(self):
         return AfqTrial(1 if self.trial_rhf else 0, dptr(self.phi_up), dptr(self.phi_dn))

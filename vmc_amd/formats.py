@@ -58,9 +58,15 @@ def load_case(path):
     left, right = str(z["left"]), str(z["right"])
     wrhf = walker_is_rhf(nalpha, nbeta, right)
     lu, ld = trial_orbitals(left, z, norb, nalpha, nbeta)
-    ham = HamArrays(norb, nalpha, nbeta, float(z["energy_core"][0]),
-                    z["hcore"], z["hcore_mod"], z["chol"], lu, ld,
-                    trial_rhf=(left == "rhf"))
+    if "chol_up" in z.files:  # unrestricted integrals (utils/integral.cpp:664-741)
+        ham = HamArrays(norb, nalpha, nbeta, float(z["energy_core"][0]),
+                        z["hcore_up"], z["hcore_mod_up"], z["chol_up"], lu, ld,
+                        trial_rhf=(left == "rhf"), h1_dn=z["hcore_dn"], h1_mod_dn=z["hcore_mod_dn"],
+                        chol_dn=z["chol_dn"])
+    else:
+        ham = HamArrays(norb, nalpha, nbeta, float(z["energy_core"][0]),
+                        z["hcore"], z["hcore_mod"], z["chol"], lu, ld,
+                        trial_rhf=(left == "rhf"))
     # reference dets: rhf.txt for an RHF walker, uhf.txt otherwise
     # (DQMC/MixedEstimator.cpp:338-351)
     ru, rd = trial_orbitals("rhf" if wrhf else "uhf", z, norb, nalpha, nbeta)
