@@ -18,7 +18,7 @@ from vmc_amd.formats import load_case
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["rhf_rhf", "uhf_uhf", "rhf_uhf", "uhf_rhf"]
+CASES = ["rhf_rhf", "uhf_uhf", "rhf_uhf", "uhf_rhf", "uhf_uhf_ui"]   # uhf_uhf_ui: unrestricted integrals
 
 
 def kw(c, **extra):

@@ -78,6 +78,21 @@ def perturbed_walkers(h, W, walker_rhf, rng, scale=0.25):
     return np.array(dets)
 
 
+def unrestricted_system(N, na, nb, G, seed):
+    """A synthetic system with unrestricted integrals (intType "u"): the UHF system of
+    make_system for spin up, and for spin down other symmetric Cholesky vectors and a
+    perturbed h1 with its own h1Mod."""
+    base = make_system(N, na, nb, G, seed=seed, trial="uhf")
+    rng = np.random.default_rng(seed + 1000)
+    Ld = rng.standard_normal((G, N, N)) * 0.3 / np.sqrt(N)
+    Ld = 0.5 * (Ld + Ld.transpose(0, 2, 1))
+    h1d = base.h1 + 0.05 * np.diag(rng.standard_normal(N))
+    h1md = h1d - 0.5 * np.einsum("gij,gjk->ik", Ld, Ld)
+    from vmc_amd.abi import HamArrays
+    return HamArrays(N, na, nb, 0.0, base.h1, base.h1_mod, base.chol, base.phi_up, base.phi_dn, trial_rhf=False,
+                     h1_dn=h1d, h1_mod_dn=h1md, chol_dn=Ld)
+
+
 SYSTEMS = {
     # name: (builder, walker_rhf)
     "rhf_rhf": (lambda: load_case(os.path.join(GOLD, "rhf_rhf.npz")), None),
@@ -88,6 +103,11 @@ SYSTEMS = {
     "uhf_33_31": (lambda: make_system(70, 33, 31, 130, seed=102, trial="uhf"), False),
     "uhf_n60": (lambda: make_system(150, 60, 60, 100, seed=103, trial="uhf"), False),
     "c1_small": (lambda: make_system(100, 20, 20, 400, seed=104, trial="uhf"), False),
+    # unrestricted integrals (propagatePhaselessU): the reference fixture and shapes on
+    # both Taylor kernels (k_vmul at <= 64 columns per determinant, k_taylor above)
+    "uhf_uhf_ui": (lambda: load_case(os.path.join(GOLD, "uhf_uhf_ui.npz")), None),
+    "u_33_31": (lambda: unrestricted_system(70, 33, 31, 130, seed=105), False),
+    "u_n60": (lambda: unrestricted_system(150, 60, 60, 100, seed=106), False),
 }
 
 

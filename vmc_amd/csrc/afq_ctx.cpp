@@ -90,6 +90,7 @@ struct afq_ctx {
   bool prepped = false;
   bool have_walkers = false;
   std::vector<double> h1, h1mod, chol;  // host copies (setup math)
+  std::vector<double> h1d, h1modd, chold;  // down-spin copies with unrestricted integrals
   std::vector<double> phi[2];
   std::vector<float> dummy;
   // stage timing
@@ -175,7 +176,7 @@ struct Stage {
 };
 
 void free_all(DevState& s) {
-  double* ptrs[] = {s.E, s.Lp, s.R, s.phi[0], s.phi[1], s.hphi[0], s.hphi[1], s.mf, s.psi, s.theta, s.fb,
+  double* ptrs[] = {s.E2, s.Lp2, s.P2, s.E, s.Lp, s.R, s.phi[0], s.phi[1], s.hphi[0], s.hphi[1], s.mf, s.psi, s.theta, s.fb,
                     s.logO, s.e1, s.logdet, s.weights, s.factors, s.psi2, s.theta2, s.fb2, s.logO2, s.e12,
                     s.fields, s.sft, s.P, s.t0, s.t1, s.accb, s.draws, s.Xp, s.eloc, s.ho, s.cum, s.Sg};
   for (double* p : ptrs)
@@ -287,6 +288,7 @@ void free_all(DevState& s);
 void free_temporary(const DevState& t) {
   DevState z = t;
   z.E = z.Lp = z.R = z.mf = nullptr;
+  z.E2 = z.Lp2 = z.P2 = nullptr;   // the context's (a temporary population does not propagate)
   z.phi[0] = z.phi[1] = z.hphi[0] = z.hphi[1] = nullptr;
   z.Rb_hi = z.Rb_lo = z.Tb_hi = z.Tb_lo = nullptr;
   z.Lt_hi = z.Lt_lo = z.Fb_hi = z.Fb_lo = nullptr;
@@ -341,29 +343,33 @@ void download(double* h, const double* d, size_t n, hipStream_t st) {
 
 // Mean-field shifts and propagator constant of DQMCWalker::prepPropR
 // (DQMC/DQMCWalker.cpp:32-103), phaseless branch; returns h1' = h1Mod - sum mf_g (i L_g).
+// With unrestricted integrals (prepPropU, :106-187) the shifts sum both spins' terms
+// and h1p2 receives the down-spin h1'.
 void prep_math(afq_ctx* c, const double* gref_up, const double* gref_dn, double dt,
-               std::vector<double>& mf, std::vector<double>& h1p, double& pc) {
+               std::vector<double>& mf, std::vector<double>& h1p, double& pc, std::vector<double>* h1p2 = nullptr) {
   DevState& s = c->s;
   const int N = s.N, G = s.G;
   const size_t NN = (size_t)N * N;
   h1p = c->h1mod;
+  if (s.uints && h1p2) *h1p2 = c->h1modd;
   mf.assign(2 * (size_t)G, 0.0);
   double constant = -s.ecore;  // ene0 cancels in the phaseless constant
+  auto rd = [&](double l) { return s.ref_compat ? (double)(float)l : l; };
   for (int g = 0; g < G; ++g) {
     const double* L = &c->chol[g * NN];
+    const double* Ld = s.uints ? &c->chold[g * NN] : nullptr;
     double a = 0.0;
-    for (size_t q = 0; q < NN; ++q) {
-      const double l = s.ref_compat ? (double)(float)L[q] : L[q];
-      a += (gref_up[q] + gref_dn[q]) * l;
-    }
+    if (s.uints)
+      for (size_t q = 0; q < NN; ++q) a += gref_up[q] * rd(L[q]) + gref_dn[q] * rd(Ld[q]);
+    else
+      for (size_t q = 0; q < NN; ++q) a += (gref_up[q] + gref_dn[q]) * rd(L[q]);
     // mf = i a;  constant -= mf^2/2 = +a^2/2;  h1' -= mf * (i L) = + a L
     mf[2 * g] = 0.0;
     mf[2 * g + 1] = a;
     constant += 0.5 * a * a;
-    for (size_t q = 0; q < NN; ++q) {
-      const double l = s.ref_compat ? (double)(float)L[q] : L[q];
-      h1p[q] += a * l;
-    }
+    for (size_t q = 0; q < NN; ++q) h1p[q] += a * rd(L[q]);
+    if (s.uints && h1p2)
+      for (size_t q = 0; q < NN; ++q) (*h1p2)[q] += a * rd(Ld[q]);
   }
   pc = constant;
   (void)dt;
@@ -544,6 +550,13 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
   if (params->ref_compat && params->prec_vhs != AFQ_PREC_FP64)
     return fail(nullptr, AFQ_ERR_UNSUPPORTED, "ref_compat rounds the VHS itself; it needs prec_vhs = FP64");
+  const bool uints = ham->h1_dn || ham->h1_mod_dn || ham->chol_dn;
+  if (uints && !(ham->h1_dn && ham->h1_mod_dn && ham->chol_dn))
+    return fail(nullptr, AFQ_ERR_ARG, "unrestricted integrals need h1_dn, h1_mod_dn and chol_dn");
+  if (uints && (params->walker_rhf || left->rhf))
+    return fail(nullptr, AFQ_ERR_UNSUPPORTED, "unrestricted integrals need a UHF trial and UHF walkers");
+  if (uints && params->prec_vhs != AFQ_PREC_FP64)
+    return fail(nullptr, AFQ_ERR_UNSUPPORTED, "unrestricted integrals need the FP64 VHS");
   afq_ctx* c = new afq_ctx();
   try {
     c->params = *params;
@@ -556,6 +569,7 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     s.walker_rhf = params->walker_rhf != 0;
     s.trial_rhf = left->rhf != 0;
     s.ref_compat = params->ref_compat != 0;
+    s.uints = uints;
     s.ecore = ham->ecore;
     s.weight_cap = params->weight_cap > 0 ? params->weight_cap : std::max(100.0, W / 10.0);
     s.det_n[0] = na; s.det_n[1] = nb;
@@ -587,6 +601,11 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     c->h1.assign(ham->h1, ham->h1 + NN);
     c->h1mod.assign(ham->h1_mod, ham->h1_mod + NN);
     c->chol.assign(ham->chol, ham->chol + (size_t)G * NN);
+    if (uints) {
+      c->h1d.assign(ham->h1_dn, ham->h1_dn + NN);
+      c->h1modd.assign(ham->h1_mod_dn, ham->h1_mod_dn + NN);
+      c->chold.assign(ham->chol_dn, ham->chol_dn + (size_t)G * NN);
+    }
     c->phi[0].assign(left->phi_up, left->phi_up + (size_t)N * na);
     c->phi[1] = left->rhf ? c->phi[0] : std::vector<double>(left->phi_dn, left->phi_dn + (size_t)N * nb);
     // device Hamiltonian / trial
@@ -595,15 +614,24 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     s.Lp = dalloc<double>((size_t)G * s.T);
     afq::launch_pack_chol(Ld, s.Lp, N, G, s.T, s.ref_compat, c->stream);
     launch_check("pack_chol");
+    double* Ldd = nullptr;  // down-spin vectors (unrestricted integrals)
+    if (uints) {
+      Ldd = dalloc<double>((size_t)G * NN);
+      upload(Ldd, c->chold.data(), (size_t)G * NN, c->stream);
+      s.Lp2 = dalloc<double>((size_t)G * s.T);
+      afq::launch_pack_chol(Ldd, s.Lp2, N, G, s.T, s.ref_compat, c->stream);
+      launch_check("pack_chol");
+    }
     s.R = dalloc<double>((size_t)G * s.Ktot);
     for (int p = 0; p < 2; ++p) {
       const int n = p == 0 ? na : nb;
       s.phi[p] = dalloc<double>((size_t)N * n);
       upload(s.phi[p], c->phi[p].data(), (size_t)N * n, c->stream);
       std::vector<double> hp((size_t)N * n, 0.0);
+      const std::vector<double>& h1s = (uints && p == 1) ? c->h1d : c->h1;  // spin p's h1
       for (int i = 0; i < N; ++i)
         for (int k = 0; k < N; ++k) {
-          const double h = c->h1[(size_t)i * N + k];
+          const double h = h1s[(size_t)i * N + k];
           for (int j = 0; j < n; ++j) hp[(size_t)i * n + j] += h * c->phi[p][(size_t)k * n + j];
         }
       s.hphi[p] = dalloc<double>((size_t)N * n);
@@ -611,12 +639,19 @@ int afq_create(const afq_ham* ham, const afq_trial* left, const afq_params* para
     }
     for (int q = 0; q < s.nslots; ++q) {
       const afq::Slot& sl = s.slot[q];
-      afq::launch_rotate_chol(Ld, s.phi[sl.phi], s.R, N, sl.n, G, s.Ktot, sl.koff, c->stream);
+      // unrestricted integrals: each spin's rows from its own vectors (DQMC/Hamiltonian.cpp:99-107)
+      afq::launch_rotate_chol((uints && sl.phi == 1) ? Ldd : Ld, s.phi[sl.phi], s.R, N, sl.n, G, s.Ktot, sl.koff,
+                              c->stream);
       launch_check("rotate_chol");
     }
     hip_check(hipStreamSynchronize(c->stream), "setup sync");
     hip_check(hipFree(Ld), "hipFree");
+    if (Ldd) hip_check(hipFree(Ldd), "hipFree");
     s.E = dalloc<double>(NN);
+    if (uints) {
+      s.E2 = dalloc<double>(NN);
+      s.P2 = dalloc<double>(2 * (size_t)W * NN);
+    }
     s.mf = dalloc<double>(2 * (size_t)G);
     alloc_walkers(s);
     afq::launch_fill(s.weights, W, 0.0, c->stream);
@@ -658,6 +693,8 @@ int afq_set_precision(afq_ctx* c, int32_t prec_fb, int32_t prec_vhs, int32_t pre
     return fail(c, AFQ_ERR_UNSUPPORTED, "requested precision mode not available");
   if (c->s.ref_compat && prec_vhs != AFQ_PREC_FP64)
     return fail(c, AFQ_ERR_UNSUPPORTED, "ref_compat rounds the VHS itself; it needs prec_vhs = FP64");
+  if (c->s.uints && prec_vhs != AFQ_PREC_FP64)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "unrestricted integrals need the FP64 VHS");
   AFQ_GUARD_BEGIN
   ensure_exchange_prec(c->s, prec_exchange, c->stream);
   if (prec_fb != AFQ_PREC_FP64) ensure_lowprec(c->s, c->stream);
@@ -678,6 +715,8 @@ int afq_set_prop_precision(afq_ctx* c, int32_t prec_prop) {
     return fail(c, AFQ_ERR_UNSUPPORTED, "propagation precision must be FP64 or BF16X3");
   if (prec_prop != AFQ_PREC_FP64 && c->s.ref_compat)
     return fail(c, AFQ_ERR_UNSUPPORTED, "ref_compat needs the FP64 propagation");
+  if (prec_prop != AFQ_PREC_FP64 && c->s.uints)
+    return fail(c, AFQ_ERR_UNSUPPORTED, "unrestricted integrals need the FP64 propagation");
   AFQ_GUARD_BEGIN
   if (prec_prop != AFQ_PREC_FP64) ensure_prop_lowprec(c->s, c->stream);
   c->s.prec_prop = prec_prop;
@@ -708,15 +747,19 @@ int afq_prep_prop(afq_ctx* c, const double* ref_up, const double* ref_dn, const 
     green_ref(ref_up, s.N, s.na, gu);
     green_ref(ref_dn, s.N, s.nb, gd);
   }
-  std::vector<double> mf, h1p;
+  std::vector<double> mf, h1p, h1p2;
   double pc = 0.0;
-  prep_math(c, gu.data(), gd.data(), dt, mf, h1p, pc);
+  prep_math(c, gu.data(), gd.data(), dt, mf, h1p, pc, &h1p2);
   const std::vector<double> E = expm_scaled(h1p, s.N, -dt / 2.0);
   s.dt = dt;
   s.prop_const_re = pc;
   s.prop_const_im = 0.0;
   upload(s.mf, mf.data(), mf.size(), c->stream);
   upload(s.E, E.data(), E.size(), c->stream);
+  if (s.uints) {
+    const std::vector<double> E2 = expm_scaled(h1p2, s.N, -dt / 2.0);
+    upload(s.E2, E2.data(), E2.size(), c->stream);
+  }
   if (s.Dh) afq::launch_d_split(s, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "prep sync");
   c->prepped = true;
@@ -1772,6 +1815,9 @@ static void batch_free(afq_batch* b) {
 int afq_batch_create(int32_t nsys, const afq_ham* hams, const afq_trial* trials, const afq_params* params,
                      afq_batch** out) {
   if (nsys <= 0 || !hams || !trials || !params || !out) return fail(nullptr, AFQ_ERR_ARG, "bad batch arguments");
+  for (int i = 0; i < nsys; ++i)
+    if (hams[i].h1_dn || hams[i].h1_mod_dn || hams[i].chol_dn)
+      return fail(nullptr, AFQ_ERR_UNSUPPORTED, "batched systems take restricted integrals only");
   afq_batch* b = new afq_batch();
   for (int i = 0; i < nsys; ++i) {
     afq_ctx* c = nullptr;

@@ -41,6 +41,12 @@ struct DevState {
   double ecore = 0.0, dt = 0.0, weight_cap = 100.0;
   double prop_const_re = 0.0, prop_const_im = 0.0;
 
+  // unrestricted integrals (intType "u"): per-spin one-body propagator, Cholesky
+  // vectors and VHS; E, Lp and P are then the up-spin ones, E2, Lp2 and P2 the
+  // down-spin ones (FP64 VHS and propagation only)
+  bool uints = false;
+  double* E2 = nullptr; double* Lp2 = nullptr; double* P2 = nullptr;
+
   // Hamiltonian / trial (device)
   double* E = nullptr;      // exp(-dt/2 h1'), N x N
   double* Lp = nullptr;     // packed lower-triangle L, G x T (FP64, or float-rounded in ref_compat)

@@ -218,6 +218,11 @@ void launch_vhs(const DevState& s, hipStream_t st) {
   const int gx = (int)cdivu(s.T, VHS_BN), gy = (int)cdivu(2 * (int64_t)s.W, VHS_BM);
   hipLaunchKernelGGL(k_vhs, dim3(gx * gy), dim3(256), 0, st, s.fields, s.Lp, s.P, s.weights, s.G, s.W, s.T,
                      s.N, (int)s.ref_compat, gx, gy);
+  // unrestricted integrals: the down-spin VHS from the down-spin vectors
+  // (propagatePhaselessU, DQMC/DQMCWalker.cpp:525-530)
+  if (s.uints)
+    hipLaunchKernelGGL(k_vhs, dim3(gx * gy), dim3(256), 0, st, s.fields, s.Lp2, s.P2, s.weights, s.G, s.W, s.T,
+                       s.N, (int)s.ref_compat, gx, gy);
 }
 
 // ------------------------------------------------------------------ propagation
@@ -1141,7 +1146,8 @@ void launch_expmul_pre(const DevState& s, hipStream_t st) {
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
     const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
-    hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.psi, s.t0, s.accb, s.weights, s.N,
+    const double* E = (s.uints && d == 1) ? s.E2 : s.E;  // spin d's propagator
+    hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, E, s.psi, s.t0, s.accb, s.weights, s.N,
                        n, s.W, ws, 2 * s.det_off[d], 0, gx, gy);
   }
 }
@@ -1177,7 +1183,44 @@ static void launch_taylor_split(const DevState& s, hipStream_t st) {
   if (v == 2) taylor_split_cols<5>(s, st);
   else taylor_split_cols<4>(s, st);
 }
+// The Taylor products of the determinants listed: with a single determinant d the
+// kernels see only its columns (a walker block offset to det d, no second part).
+static void taylor_fp64(const DevState& s, const double* P, int dsel, hipStream_t st) {
+  const int64_t ws = 2 * s.ms;
+  const int na = dsel < 0 ? s.det_n[0] : s.det_n[dsel];
+  const int nb = dsel < 0 && s.ndets_prop > 1 ? s.det_n[1] : 0;
+  const int n2a = 2 * na, n2b = 2 * nb;
+  const int64_t offb = 2 * s.det_off[1];
+  const int64_t base = dsel > 0 ? 2 * s.det_off[dsel] : 0;
+  const double sq = sqrt(s.dt);
+  double* src = s.t0 + base;
+  double* dst = s.t1 + base;
+  double* acc = s.accb + base;
+  const bool wide = (s.taylor_variant == 1 || s.taylor_variant == 2) ? s.taylor_variant == 2 : n2a + n2b > 128;
+  for (int p = 1; p <= 5; ++p) {
+    if (wide) {
+      const int gx = (int)cdivu(na + nb, TY_BNC), gy = (int)cdivu(s.N, TY_BM);
+      if (s.N % 2 == 0)
+        hipLaunchKernelGGL(k_taylor<true>, dim3(gx * gy * s.W), dim3(512), 0, st, P, src, dst, acc, s.weights,
+                           s.N, na, nb, s.det_off[1], ws, sq / p, gx, gy, s.W);
+      else
+        hipLaunchKernelGGL(k_taylor<false>, dim3(gx * gy * s.W), dim3(512), 0, st, P, src, dst, acc, s.weights,
+                           s.N, na, nb, s.det_off[1], ws, sq / p, gx, gy, s.W);
+    } else {
+      const int gx = (int)cdivu(n2a + n2b, 128), gy = (int)cdivu(s.N, 64);
+      hipLaunchKernelGGL((k_vmul<64, 128, 2, 2>), dim3(gx * gy * s.W), dim3(256), 0, st, P, src, dst, acc,
+                         s.weights, s.N, s.T, n2a, n2b, offb, ws, sq, 1.0 / p, gx, gy, s.W);
+    }
+    double* t = src; src = dst; dst = t;
+  }
+}
 void launch_taylor(const DevState& s, hipStream_t st) {
+  if (s.uints && s.prec_prop == AFQ_PREC_FP64) {
+    // unrestricted integrals: each determinant with its own spin's V (DQMC/DQMCWalker.cpp:532-540)
+    taylor_fp64(s, s.P, 0, st);
+    taylor_fp64(s, s.P2, 1, st);
+    return;
+  }
   const int64_t ws = 2 * s.ms;
   const int n2a = 2 * s.det_n[0];
   const int n2b = s.ndets_prop > 1 ? 2 * s.det_n[1] : 0;
@@ -1218,7 +1261,8 @@ void launch_expmul_post(const DevState& s, hipStream_t st) {
   for (int d = 0; d < s.ndets_prop; ++d) {
     const int n = s.det_n[d];
     const int gx = (int)cdivu((int64_t)s.W * 2 * n, EX_BN), gy = (int)cdivu(s.N, EX_BM);
-    hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, s.E, s.accb, s.psi, nullptr, s.weights,
+    const double* E = (s.uints && d == 1) ? s.E2 : s.E;
+    hipLaunchKernelGGL(k_expmul, dim3(gx * gy), dim3(256), 0, st, E, s.accb, s.psi, nullptr, s.weights,
                        s.N, n, s.W, ws, 2 * s.det_off[d], 1, gx, gy);
   }
 }
