@@ -32,10 +32,23 @@ int afq_fcidump_header(const char* path, int64_t* header);
  * = /energy_core, or 0 when that dataset is absent (as the reference). */
 int afq_fcidump_read(const char* path, double* h1, double* h1_mod, double* chol, double* ecore);
 
+/* readDQMCIntegralsU (utils/integral.cpp:664-741), unrestricted integrals:
+ * /hcore_up, /hcore_dn, /hcore_mod_up, /hcore_mod_dn (norb x norb row-major) and
+ * /chol_up, /chol_dn (nchol x norb x norb).  Returns AFQ_IO_ERR_DATASET for a
+ * restricted file. */
+int afq_fcidump_read_u(const char* path, double* h1_up, double* h1_dn, double* h1_mod_up, double* h1_mod_dn,
+                       double* chol_up, double* chol_dn, double* ecore);
+
 /* write_dqmc (scripts/QMCUtils.py:1067-1075): the same datasets, types and
  * shapes the reference's writer produces. */
 int afq_fcidump_write(const char* path, int64_t nelec, int64_t norb, int64_t ms, int64_t nchol,
                       const double* h1, const double* h1_mod, const double* chol, double ecore);
+
+/* write_dqmc_uihf (scripts/QMCUtils.py:1088-1098): unrestricted integrals, the
+ * datasets afq_fcidump_read_u reads. */
+int afq_fcidump_write_u(const char* path, int64_t nelec, int64_t norb, int64_t ms, int64_t nchol,
+                        const double* h1_up, const double* h1_dn, const double* h1_mod_up,
+                        const double* h1_mod_dn, const double* chol_up, const double* chol_dn, double ecore);
 
 #ifdef __cplusplus
 }

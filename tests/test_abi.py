@@ -66,6 +66,7 @@ def test_io_library_exports_its_header():
         pytest.skip("libvmc_fcidump.so not built (no HDF5 C library)")
     text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "vmc_afqmc_io.h")).read(), flags=re.S)
     names = sorted(set(re.findall(r"^\s*\w+\s*\*?\s*(afq_\w+)\s*\(", text, flags=re.M)))
-    assert names == ["afq_fcidump_header", "afq_fcidump_read", "afq_fcidump_write"]
+    assert names == ["afq_fcidump_header", "afq_fcidump_read", "afq_fcidump_read_u", "afq_fcidump_write",
+                     "afq_fcidump_write_u"]
     lib = C.CDLL(io_lib)
     assert all(hasattr(lib, n) for n in names)

@@ -91,6 +91,25 @@ def test_fcidump_round_trip(tmp_path):
 
 
 @io_built
+def test_fcidump_unrestricted_round_trip(tmp_path):
+    from vmc_amd.formats import write_fcidump_chol_u
+    rng = np.random.default_rng(5)
+    N, G = 4, 3
+    h1 = [rng.standard_normal((N, N)) for _ in range(2)]
+    h1m = [rng.standard_normal((N, N)) for _ in range(2)]
+    L = [rng.standard_normal((G, N, N)) for _ in range(2)]
+    p = str(tmp_path / "FCIDUMP_chol")
+    write_fcidump_chol_u(p, h1, h1m, L, nelec=5, ecore=-1.5, ms=1)
+    d = read_fcidump_chol(p, "u")
+    assert (d.nelec, d.norb, d.nchol, d.nalpha, d.nbeta) == (5, N, G, 3, 2)
+    assert np.array_equal(d.h1, h1[0]) and np.array_equal(d.h1_dn, h1[1])
+    assert np.array_equal(d.h1_mod, h1m[0]) and np.array_equal(d.h1_mod_dn, h1m[1])
+    assert np.array_equal(d.chol, L[0]) and np.array_equal(d.chol_dn, L[1]) and d.ecore == -1.5
+    with pytest.raises(RuntimeError):
+        read_fcidump_chol(p)   # no restricted datasets in an unrestricted file
+
+
+@io_built
 def test_fcidump_missing_file(tmp_path):
     with pytest.raises(RuntimeError):
         read_fcidump_chol(str(tmp_path / "absent"))
@@ -105,8 +124,14 @@ def test_fcidump_reads_reference_files(name):
     if not os.path.exists(path):
         pytest.skip("reference test files not present")
     z = np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
-    if "hcore_mod" not in z.files:
-        pytest.skip("fixture holds no restricted integrals")
+    if "hcore_mod" not in z.files:   # unrestricted integrals (utils/integral.cpp:664-741)
+        d = read_fcidump_chol(path, "u")
+        assert [d.nelec, d.norb, d.ms, d.nchol] == [int(v) for v in z["header"]]
+        for got, key in ((d.h1, "hcore_up"), (d.h1_dn, "hcore_dn"), (d.h1_mod, "hcore_mod_up"),
+                         (d.h1_mod_dn, "hcore_mod_dn"), (d.chol, "chol_up"), (d.chol_dn, "chol_dn")):
+            assert np.array_equal(got.ravel(), np.asarray(z[key]).ravel()), key
+        assert d.ecore == float(np.asarray(z["energy_core"]).ravel()[0])
+        return
     d = read_fcidump_chol(path)
     assert [d.nelec, d.norb, d.ms, d.nchol] == [int(v) for v in z["header"]]
     assert np.array_equal(d.h1.ravel(), np.asarray(z["hcore"]).ravel())

@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from vmc_amd.formats import load_case, write_fcidump_chol
+from vmc_amd.formats import load_case, write_fcidump_chol, write_fcidump_chol_u
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -24,14 +24,18 @@ def make_directory(tmp_path, name, write_rdm=False):
     h = c.ham
     nelec, N, ms, G = (int(v) for v in z["header"])
     try:
-        write_fcidump_chol(str(tmp_path / "FCIDUMP_chol"), h.h1, h.h1_mod, h.chol, nelec=nelec,
-                           ecore=h.ecore, ms=ms)
+        if h.unrestricted:
+            write_fcidump_chol_u(str(tmp_path / "FCIDUMP_chol"), (h.h1, h.h1_dn), (h.h1_mod, h.h1_mod_dn),
+                                 (h.chol, h.chol_dn), nelec=nelec, ecore=h.ecore, ms=ms)
+        else:
+            write_fcidump_chol(str(tmp_path / "FCIDUMP_chol"), h.h1, h.h1_mod, h.chol, nelec=nelec,
+                               ecore=h.ecore, ms=ms)
     except (OSError, RuntimeError) as e:
         pytest.skip("FCIDUMP_chol writer unavailable: %s" % e)
     for kind in ("rhf", "uhf"):
         if kind in z.files:
             np.savetxt(str(tmp_path / (kind + ".txt")), np.asarray(z[kind]), fmt="%.17g")
-    cfg = {"system": {"integrals": "FCIDUMP_chol"},
+    cfg = {"system": {"integrals": "FCIDUMP_chol", "intType": "u" if h.unrestricted else "r"},
            "wavefunction": {"left": c.left, "right": c.right},
            "sampling": {"seed": c.seed, "phaseless": True, "dt": c.dt, "nsteps": c.nsteps, "nwalk": c.nwalk,
                         "stochasticIter": c.nsweeps, "choleskyThreshold": c.cholesky_threshold,
@@ -41,7 +45,7 @@ def make_directory(tmp_path, name, write_rdm=False):
     return c
 
 
-@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_uhf"])
+@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_uhf", "uhf_uhf_ui"])
 def test_run_directory_reproduces_samples_ref(tmp_path, name):
     from vmc_amd.dqmc import run_directory
     c = make_directory(tmp_path, name)

@@ -1,5 +1,6 @@
-// FCIDUMP_chol (HDF5) reader and writer: readDQMCIntegralsRG (utils/integral.cpp:589-661)
-// and write_dqmc (scripts/QMCUtils.py:1067-1075), host-side over the HDF5 C library.
+// FCIDUMP_chol (HDF5) readers and writers: readDQMCIntegralsRG / readDQMCIntegralsU
+// (utils/integral.cpp:589-741) and write_dqmc / write_dqmc_uihf (scripts/QMCUtils.py:1067-1098),
+// host-side over the HDF5 C library.
 #include <hdf5.h>
 
 #include <cstdint>
@@ -106,6 +107,31 @@ extern "C" int afq_fcidump_read(const char* path, double* h1, double* h1_mod, do
   return AFQ_IO_OK;
 }
 
+extern "C" int afq_fcidump_read_u(const char* path, double* h1_up, double* h1_dn, double* h1_mod_up,
+                                  double* h1_mod_dn, double* chol_up, double* chol_dn, double* ecore) {
+  File f;
+  int rc = open_ro(path, f);
+  if (rc) return rc;
+  int64_t hd[4];
+  if ((rc = read_dataset(f.id, "/header", H5T_NATIVE_INT64, hd, 4))) return rc;
+  const int64_t N = hd[1], G = hd[3];
+  if (N <= 0 || G < 0) return AFQ_IO_ERR_SHAPE;
+  struct Item { const char* name; double* out; int64_t n; };
+  const Item items[6] = {{"/hcore_up", h1_up, N * N},         {"/hcore_dn", h1_dn, N * N},
+                         {"/hcore_mod_up", h1_mod_up, N * N}, {"/hcore_mod_dn", h1_mod_dn, N * N},
+                         {"/chol_up", chol_up, G * N * N},    {"/chol_dn", chol_dn, G * N * N}};
+  for (const Item& it : items)
+    if (it.out && (rc = read_dataset(f.id, it.name, H5T_NATIVE_DOUBLE, it.out, it.n))) return rc;
+  if (ecore) {
+    *ecore = 0.0;
+    if (dataset_size(f.id, "/energy_core") == 1) {
+      double e = 0.0;
+      if (read_dataset(f.id, "/energy_core", H5T_NATIVE_DOUBLE, &e, 1) == AFQ_IO_OK) *ecore = e;
+    }
+  }
+  return AFQ_IO_OK;
+}
+
 extern "C" int afq_fcidump_write(const char* path, int64_t nelec, int64_t norb, int64_t ms, int64_t nchol,
                                  const double* h1, const double* h1_mod, const double* chol, double ecore) {
   if (!path || !h1 || !h1_mod || (!chol && nchol > 0) || norb <= 0 || nchol < 0) return AFQ_IO_ERR_ARG;
@@ -119,5 +145,28 @@ extern "C" int afq_fcidump_write(const char* path, int64_t nelec, int64_t norb, 
   if ((rc = write_dataset(f.id, "hcore", H5T_IEEE_F64LE, h1, NN))) return rc;
   if ((rc = write_dataset(f.id, "hcore_mod", H5T_IEEE_F64LE, h1_mod, NN))) return rc;
   if ((rc = write_dataset(f.id, "chol", H5T_IEEE_F64LE, chol, nchol * NN))) return rc;
+  return write_dataset(f.id, "energy_core", H5T_IEEE_F64LE, &ecore, 0);
+}
+
+extern "C" int afq_fcidump_write_u(const char* path, int64_t nelec, int64_t norb, int64_t ms, int64_t nchol,
+                                   const double* h1_up, const double* h1_dn, const double* h1_mod_up,
+                                   const double* h1_mod_dn, const double* chol_up, const double* chol_dn,
+                                   double ecore) {
+  if (!path || !h1_up || !h1_dn || !h1_mod_up || !h1_mod_dn || norb <= 0 || nchol < 0 ||
+      (nchol > 0 && (!chol_up || !chol_dn)))
+    return AFQ_IO_ERR_ARG;
+  File f;
+  f.id = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (f.id < 0) return AFQ_IO_ERR_OPEN;
+  const int64_t hd[4] = {nelec, norb, ms, nchol};
+  const int64_t NN = norb * norb;
+  int rc;
+  if ((rc = write_dataset(f.id, "header", H5T_STD_I64LE, hd, 4))) return rc;
+  if ((rc = write_dataset(f.id, "hcore_up", H5T_IEEE_F64LE, h1_up, NN))) return rc;
+  if ((rc = write_dataset(f.id, "hcore_dn", H5T_IEEE_F64LE, h1_dn, NN))) return rc;
+  if ((rc = write_dataset(f.id, "hcore_mod_up", H5T_IEEE_F64LE, h1_mod_up, NN))) return rc;
+  if ((rc = write_dataset(f.id, "hcore_mod_dn", H5T_IEEE_F64LE, h1_mod_dn, NN))) return rc;
+  if ((rc = write_dataset(f.id, "chol_up", H5T_IEEE_F64LE, chol_up, nchol * NN))) return rc;
+  if ((rc = write_dataset(f.id, "chol_dn", H5T_IEEE_F64LE, chol_dn, nchol * NN))) return rc;
   return write_dataset(f.id, "energy_core", H5T_IEEE_F64LE, &ecore, 0);
 }

@@ -3,7 +3,7 @@
 Reads what executables/DQMC.cpp reads -- the JSON input (utils/input.cpp:61-276,
 the keys the phaseless walk uses, with the reference's defaults; keys are
 case-insensitive as in its boost iptree), the FCIDUMP_chol integrals
-(utils/integral.cpp:589-661) and the rhf.txt / uhf.txt trial text
+(utils/integral.cpp:589-661, or :664-741 for system.intType "u") and the rhf.txt / uhf.txt trial text
 (utils/input.cpp:376-383), and spinRDM.0.0.txt where present (the background
 density, DQMC/DQMCWalker.cpp:42-58) -- picks the walker type and trials as
 executables/DQMC.cpp:74-185 does, runs calcMixedEstimatorLongProp on the GPU
@@ -93,13 +93,13 @@ def run_directory(directory, input_file="afqmc.json", *, nranks=1, ref_compat=Tr
     from .afqmc import run_long_prop
 
     s = read_afqmc_json(os.path.join(directory, input_file))
-    if s.int_type != "r":
-        raise NotImplementedError("intType %r: only restricted integrals are on this path" % s.int_type)
+    if s.int_type not in ("r", "u"):
+        raise NotImplementedError("intType %r: restricted or unrestricted integrals only" % s.int_type)
     if not s.phaseless:
         raise NotImplementedError("only the phaseless walk (sampling.phaseless) is on this path")
     if s.left not in ("rhf", "uhf") or s.right not in ("rhf", "uhf"):
         raise NotImplementedError("trials %s/%s: only rhf and uhf are on this path" % (s.left, s.right))
-    d = read_fcidump_chol(os.path.join(directory, s.integrals))
+    d = read_fcidump_chol(os.path.join(directory, s.integrals), s.int_type)
     N, na, nb = d.norb, d.nalpha, d.nbeta
     mats = {}
     for kind, ncol in (("rhf", N), ("uhf", 2 * N)):
@@ -108,7 +108,8 @@ def run_directory(directory, input_file="afqmc.json", *, nranks=1, ref_compat=Tr
             mats[kind] = read_text_matrix(p, N, ncol)
     wrhf = walker_is_rhf(na, nb, s.right)
     lu, ld = trial_orbitals(s.left, mats, N, na, nb)
-    ham = HamArrays(N, na, nb, d.ecore, d.h1, d.h1_mod, d.chol, lu, ld, trial_rhf=(s.left == "rhf"))
+    ham = HamArrays(N, na, nb, d.ecore, d.h1, d.h1_mod, d.chol, lu, ld, trial_rhf=(s.left == "rhf"),
+                    h1_dn=d.h1_dn, h1_mod_dn=d.h1_mod_dn, chol_dn=d.chol_dn)
     # reference dets: rhf.txt for an RHF walker, uhf.txt otherwise (DQMC/MixedEstimator.cpp:338-351)
     ru, rd = trial_orbitals("rhf" if wrhf else "uhf", mats, N, na, nb)
     iu, idn = trial_orbitals(s.right, mats, N, na, nb)

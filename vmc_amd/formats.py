@@ -152,6 +152,9 @@ def _io_lib():
         DP, LP = C.POINTER(C.c_double), C.POINTER(C.c_int64)
         L.afq_fcidump_header.argtypes = [C.c_char_p, LP]
         L.afq_fcidump_read.argtypes = [C.c_char_p, DP, DP, DP, DP]
+        L.afq_fcidump_read_u.argtypes = [C.c_char_p, DP, DP, DP, DP, DP, DP, DP]
+        L.afq_fcidump_write_u.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, DP, DP, DP, DP,
+                                          DP, DP, C.c_double]
         L.afq_fcidump_write.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, DP, DP, DP,
                                         C.c_double]
         _io = L
@@ -166,15 +169,19 @@ class Integrals:
     nchol: int
     nalpha: int
     nbeta: int
-    h1: np.ndarray       # norb x norb
+    h1: np.ndarray       # norb x norb (up spin with unrestricted integrals)
     h1_mod: np.ndarray   # norb x norb
     chol: np.ndarray     # nchol x norb x norb
     ecore: float
+    h1_dn: np.ndarray = None       # unrestricted integrals only
+    h1_mod_dn: np.ndarray = None
+    chol_dn: np.ndarray = None
 
 
-def read_fcidump_chol(path):
-    """readDQMCIntegralsRG (utils/integral.cpp:589-661): the reference's Cholesky
-    integral file; nalpha = (nelec + ms) / 2 (:614-616)."""
+def read_fcidump_chol(path, int_type="r"):
+    """readDQMCIntegralsRG (utils/integral.cpp:589-661), or readDQMCIntegralsU
+    (:664-741) for int_type "u": the reference's Cholesky integral file;
+    nalpha = (nelec + ms) / 2 (:614-616)."""
     L = _io_lib()
     hd = np.zeros(4, np.int64)
     rc = L.afq_fcidump_header(path.encode(), hd.ctypes.data_as(C.POINTER(C.c_int64)))
@@ -183,11 +190,30 @@ def read_fcidump_chol(path):
     nelec, N, ms, G = (int(v) for v in hd)
     h1, h1m, chol, ec = np.zeros((N, N)), np.zeros((N, N)), np.zeros((G, N, N)), np.zeros(1)
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    na = (nelec + ms) // 2
+    if int_type == "u":
+        h1d, h1md, chold = np.zeros((N, N)), np.zeros((N, N)), np.zeros((G, N, N))
+        rc = L.afq_fcidump_read_u(path.encode(), dp(h1), dp(h1d), dp(h1m), dp(h1md), dp(chol), dp(chold), dp(ec))
+        if rc:
+            raise RuntimeError("afq_fcidump_read_u(%s): error %d" % (path, rc))
+        return Integrals(nelec, N, ms, G, na, nelec - na, h1, h1m, chol, float(ec[0]), h1d, h1md, chold)
     rc = L.afq_fcidump_read(path.encode(), dp(h1), dp(h1m), dp(chol), dp(ec))
     if rc:
         raise RuntimeError("afq_fcidump_read(%s): error %d" % (path, rc))
-    na = (nelec + ms) // 2
     return Integrals(nelec, N, ms, G, na, nelec - na, h1, h1m, chol, float(ec[0]))
+
+
+def write_fcidump_chol_u(path, h1, h1_mod, chol, nelec, ecore, ms=0):
+    """write_dqmc_uihf (scripts/QMCUtils.py:1088-1098): h1, h1_mod, chol are
+    (up, down) pairs."""
+    N = np.asarray(h1[0]).shape[0]
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (h1[0], h1[1], h1_mod[0], h1_mod[1])]
+    ch = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1, N * N) for x in chol]
+    dp = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+    rc = _io_lib().afq_fcidump_write_u(path.encode(), int(nelec), N, int(ms), ch[0].shape[0], *[dp(x) for x in a],
+                                       dp(ch[0]), dp(ch[1]), float(ecore))
+    if rc:
+        raise RuntimeError("afq_fcidump_write_u(%s): error %d" % (path, rc))
 
 
 def write_fcidump_chol(path, h1, h1_mod, chol, nelec, ecore, ms=0):
