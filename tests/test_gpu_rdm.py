@@ -8,7 +8,8 @@ updates a running average, the device a plain sum, so they differ by rounding
 only).  Shapes: the four reference fixtures (RHF-RHF is one slot of
 multiplicity 2), a synthetic nalpha != nbeta system and the N=100 system, with
 walker counts below and above the accumulation's 32 walker chunks.  The driver
-test compares whole runs of the reference fixtures, where the 1-RDM is
+test compares whole runs of the reference fixtures (uhf_uhf_ui with unrestricted
+integrals, where the reference writes the per-spin pair as here), where the 1-RDM is
 accumulated once step dt > 10; both sides run the reference's FP32 VHS
 (ref_compat), where whole trajectories agree to the reference's own test
 tolerance, 1e-7 (tests/runDQMC.sh:9), and so does the 1-RDM.
@@ -63,7 +64,7 @@ def test_one_rdm_empty_and_reset():
     assert cw == 0.0 and not r.any()
 
 
-@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_uhf"])
+@pytest.mark.parametrize("name", ["rhf_rhf", "uhf_uhf", "uhf_uhf_ui"])
 def test_driver_one_rdm_matches_oracle_driver(name):
     c = load_case(os.path.join(GOLD, name + ".npz"))
     N = c.ham.norb
