@@ -183,6 +183,7 @@ void free_all(DevState& s) {
     if (p) (void)hipFree(p);
   if (s.idx) (void)hipFree(s.idx);
   if (s.live) (void)hipFree(s.live);
+  if (s.gfresh) (void)hipFree(s.gfresh);
   if (s.nlive) (void)hipFree(s.nlive);
   if (s.vhs_blk) (void)hipFree(s.vhs_blk);
   if (s.ncap) (void)hipFree(s.ncap);
@@ -329,6 +330,7 @@ void alloc_walkers(DevState& s) {
   s.cum = dalloc<double>(W);
   s.idx = dalloc<int>(W);
   s.live = dalloc<int>(W);
+  s.gfresh = dalloc<int>(W);
   s.nlive = dalloc<int>(1);
   s.ncap = dalloc<unsigned long long>(1);
 }
@@ -863,15 +865,20 @@ int afq_propagate(afq_ctx* c, const double* draws, uint64_t seed, uint64_t step,
   }
   {
     Stage t(c, afq::ST_GREEN);
-    // after a comb every walker's theta is recomputed, propagated or not
-    afq::launch_green(s, s.theta_stale, c->stream);
+    // after a comb every walker's overlap matrix is recomputed, propagated or not
+    afq::launch_green_pre(s, s.theta_stale, c->stream);
     launch_check("green");
-    s.theta_stale = false;
   }
   {
     Stage t(c, afq::ST_WEIGHT);
     afq::launch_weight(s, eshift, true, c->stream);
     launch_check("weight");
+  }
+  {
+    Stage t(c, afq::ST_GREEN);
+    afq::launch_theta_post(s, c->stream);   // theta of the walkers still alive
+    launch_check("theta");
+    s.theta_stale = false;
   }
   {
     Stage t(c, afq::ST_FB);

@@ -82,6 +82,7 @@ struct DevState {
   int* idx = nullptr;        // W (comb)
   int* live = nullptr;       // W: walkers of nonzero weight in ascending order, count in *nlive
   int* nlive = nullptr;
+  int* gfresh = nullptr;     // W: 1 where the last launch_green_pre took the walker
   unsigned long long* ncap = nullptr;
 
   // mixed precision
@@ -180,6 +181,8 @@ void launch_expmul_post(const DevState& s, hipStream_t st);
 bool prop_fused_fits(const DevState& s);                              // k_prop_split handles N
 void launch_prop_split_to(const DevState& s, double* dst, hipStream_t st);  // split path, result in dst
 void launch_green(const DevState& s, bool all_walkers, hipStream_t st);
+void launch_green_pre(const DevState& s, bool all_walkers, hipStream_t st);   // S, S^-1, log det S
+void launch_theta_post(const DevState& s, hipStream_t st);                   // theta, e1 of the live walkers
 void launch_weight(const DevState& s, double eshift, bool update_weights, hipStream_t st);
 void launch_logO_from_logdet(const DevState& s, hipStream_t st);
 void launch_qr(const DevState& s, hipStream_t st);

@@ -1285,6 +1285,9 @@ constexpr int GN_MAX = 64;
 //     partial e1 = sum theta o (h1 Phi).
 struct GreenArgs {
   const double* psi; double* theta; double* logdet; double* e1; const double* weights; double* Sg;
+  // fresh[w] = 1 when k_invert took walker w (its S^-1 in Sg is current); with
+  // post = 1, k_theta_r takes the fresh walkers of nonzero weight only
+  int* fresh = nullptr; int post = 0;
   const double* phi0; const double* phi1; const double* hphi0; const double* hphi1;
   int64_t ms, Ktot; int64_t det_off0, det_off1; int N, nslots, W; Slot s0, s1; int all;
 };
@@ -1324,7 +1327,9 @@ __global__ __launch_bounds__(64 * (64 / CPW), CPW == 16 ? 3 : 2) void k_invert(G
   __shared__ cplx pvs[GN_MAX];
   __shared__ int pk[GN_MAX];
   const int w = blockIdx.x, si = blockIdx.y;
-  if (!a.all && a.weights[w] <= 1e-8) return;
+  const bool take = a.all || a.weights[w] > 1e-8;
+  if (a.fresh && si == 0 && threadIdx.x == 0) a.fresh[w] = take ? 1 : 0;
+  if (!take) return;
   const Slot sl = si == 0 ? a.s0 : a.s1;
   const int n = sl.n, tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
   double* S = a.Sg + ((int64_t)w * a.nslots + si) * SG_MAT;
@@ -1423,7 +1428,7 @@ __global__ __launch_bounds__(TR_NT, 2) void k_theta_r(GreenArgs a) {
   __shared__ double As[2][2 * GN_MAX * TR_LDA];
   __shared__ double red[16];
   const int w = blockIdx.x, si = blockIdx.y;
-  if (!a.all && a.weights[w] <= 1e-8) return;
+  if (a.post ? (!a.fresh[w] || a.weights[w] == 0.0) : (!a.all && a.weights[w] <= 1e-8)) return;
   const Slot sl = si == 0 ? a.s0 : a.s1;
   const int n = sl.n, N = a.N, n2 = 2 * n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1509,13 +1514,39 @@ __global__ __launch_bounds__(TR_NT, 2) void k_theta_r(GreenArgs a) {
   }
 }
 
-void launch_green(const DevState& s, bool all, hipStream_t st) {
-  s.tb_live = false;
+static GreenArgs green_args(const DevState& s, bool all) {
   GreenArgs a;
   a.psi = s.psi; a.theta = s.theta; a.logdet = s.logdet; a.e1 = s.e1; a.weights = s.weights; a.Sg = s.Sg;
   a.phi0 = s.phi[0]; a.phi1 = s.phi[1]; a.hphi0 = s.hphi[0]; a.hphi1 = s.hphi[1];
   a.ms = s.ms; a.Ktot = s.Ktot; a.det_off0 = s.det_off[0]; a.det_off1 = s.det_off[1];
   a.N = s.N; a.nslots = s.nslots; a.W = s.W; a.s0 = s.slot[0]; a.s1 = s.slot[s.nslots > 1 ? 1 : 0]; a.all = all;
+  return a;
+}
+// S = Phi^T psi and S^-1 / log det S of the walkers above the propagation threshold
+// (every walker with all), marking them in s.gfresh
+void launch_green_pre(const DevState& s, bool all, hipStream_t st) {
+  s.tb_live = false;
+  GreenArgs a = green_args(s, all);
+  a.fresh = s.gfresh;
+  for (int si = 0; si < s.nslots; ++si) {
+    const int gx = (int)cdivu((int64_t)s.W * 2 * s.slot[si].n, 128);
+    hipLaunchKernelGGL(k_overlap, dim3(gx), dim3(256), 0, st, a, si, gx);
+  }
+  hipLaunchKernelGGL(k_invert<16>, dim3(s.W, s.nslots), dim3(256), 0, st, a);   // 4 waves: faster than 8 at n = 60
+}
+// theta = psi S^-1 (and e1) after the weight update, for the walkers launch_green_pre
+// took that are still alive: the others are never read before the next step recomputes
+// them (a walker of weight zero is neither measured nor given a force bias, and the comb
+// replaces it)
+void launch_theta_post(const DevState& s, hipStream_t st) {
+  GreenArgs a = green_args(s, false);
+  a.fresh = s.gfresh;
+  a.post = 1;
+  hipLaunchKernelGGL(k_theta_r, dim3(s.W, s.nslots), dim3(TR_NT), 0, st, a);
+}
+void launch_green(const DevState& s, bool all, hipStream_t st) {
+  s.tb_live = false;
+  const GreenArgs a = green_args(s, all);
   for (int si = 0; si < s.nslots; ++si) {
     const int gx = (int)cdivu((int64_t)s.W * 2 * s.slot[si].n, 128);
     hipLaunchKernelGGL(k_overlap, dim3(gx), dim3(256), 0, st, a, si, gx);
