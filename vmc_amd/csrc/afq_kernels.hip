@@ -1283,6 +1283,8 @@ constexpr int GN_MAX = 64;
 //     columns CPW q .. CPW q + CPW - 1), so a workgroup needs only 3 KB of LDS;
 //  3. k_theta_r: theta = psi S^-1 (FP64 MFMA, S^-1 held in registers) and the one-body
 //     partial e1 = sum theta o (h1 Phi).
+// In a propagation step 1-2 run before the weight update (the overlap ratio needs
+// log det S of every propagated walker) and 3 after it, for the walkers still alive.
 struct GreenArgs {
   const double* psi; double* theta; double* logdet; double* e1; const double* weights; double* Sg;
   // fresh[w] = 1 when k_invert took walker w (its S^-1 in Sg is current); with
