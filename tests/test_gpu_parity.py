@@ -356,3 +356,30 @@ def test_background_rdm_step(name):
     assert rel(fg, fo) < TOL, rel(fg, fo)
     check_state(h, ctx, orc, what="background rdm")
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["uhf_uhf", "rhf_rhf"])
+def test_revived_walkers_measured_from_current_orbitals(name):
+    """A step skips the Green's function of the walkers it kills; walkers given weight
+    again through afq_set_weights are then measured from their propagated orbitals."""
+    W = 5
+    h, wrhf, ctx, orc = build(name, W)
+    rng = np.random.default_rng(31)
+    dets = perturbed_walkers(h, W, wrhf, rng, 0.1)
+    ctx.set_walkers(dets)
+    orc.set_walkers(dets)
+    x = rng.standard_normal((W, h.nchol))
+    eshift = -1.0e4   # exp(dt (eshift + c)) sends every factor below 1e-3: all walkers die
+    fg, _ = ctx.propagate(x, eshift)
+    fo, _ = orc.propagate(x, eshift, 100.0)
+    assert not np.any(fg) and not np.any(fo)
+    ctx.set_weights(np.ones(W))
+    e, ho = ctx.measure()
+    dg, _, _ = ctx.get_walkers()
+    do, _, _ = orc.get_walkers()
+    assert rel(dg, do) < TOL
+    orc.set_walkers(do)
+    for w in range(W):
+        o = orc.ham_and_overlap(w)
+        assert abs(e[w] - (o[0] / o[1]).real) < TOL * abs(e[w]), w
+    ctx.close()

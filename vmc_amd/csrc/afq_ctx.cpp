@@ -440,14 +440,21 @@ void refresh_walkers(afq_ctx* c, bool overlaps_from_dets) {
   launch_check("force_bias");
 }
 
-// theta of every walker after an eager comb (which does not carry it), for the entry
-// points that read it before the next propagation
+// theta of every walker after an eager comb (which does not carry it) or new weights,
+// for the entry points that read it before the next propagation; the force bias of the
+// walkers of nonzero weight with it (a step computes it only for the walkers it leaves
+// alive, and new weights may revive others)
 void ensure_theta(afq_ctx* c) {
   DevState& s = c->s;
   if (!s.theta_stale) return;
-  Stage t(c, afq::ST_GREEN);
-  afq::launch_green(s, true, c->stream);
-  launch_check("green");
+  {
+    Stage t(c, afq::ST_GREEN);
+    afq::launch_green(s, true, c->stream);
+    launch_check("green");
+  }
+  Stage t(c, afq::ST_FB);
+  run_force_bias(s, c->stream);
+  launch_check("force_bias");
   s.theta_stale = false;
 }
 
@@ -825,6 +832,10 @@ int afq_set_weights(afq_ctx* c, const double* weights) {
   AFQ_GUARD_BEGIN
   upload(c->s.weights, weights, c->s.W, c->stream);
   c->s.tb_live = false;
+  // a walker of weight zero may carry a theta older than its orbitals (a step skips the
+  // theta of walkers it kills); new weights can make it live again, so the next reader
+  // recomputes theta for every walker
+  c->s.theta_stale = true;
   hip_check(hipStreamSynchronize(c->stream), "sync");
   return AFQ_OK;
   AFQ_GUARD_END(c)
